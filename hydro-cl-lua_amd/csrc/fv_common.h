@@ -1,0 +1,161 @@
+// fv_common.h — device-side building blocks shared by every stage kernel:
+// the kernel argument block, boundary-mapped loads of the structure-of-arrays
+// state, and 64-lane DPP neighbour exchange.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hydro_hip.h"
+
+namespace hydro {
+
+constexpr int NG = 2;              // numGhost (gridsolver.lua:41)
+constexpr int WAVE = 64;           // CDNA wavefront
+constexpr int STRIP_OUT = WAVE - 4;// output columns per wave (2-cell halo each side)
+
+// Device-resident per-solver scalars: dt minima rotate through three slots so
+// a step can read one, reduce into the next and clear the third.
+struct DevScalars {
+	double dtmin[3];
+	double t;
+	double dt;
+	double pad[3];
+};
+
+// Boundary description of one axis, resolved on the host.
+struct AxisBC {
+	int n;            // interior cells
+	int kind[2];      // HYDRO_BC_* at min / max
+};
+
+template <typename R>
+struct StageArgs {
+	// state: q integrated planes, each nx*ny*nz (x fastest), plane stride ps
+	const R *src;             // U of the stage whose derivative is taken (halo reads)
+	R *out;                   // new stage state (interior writes)
+	const R *stat;            // static (non-integrated) planes, GLM-Maxwell only
+	const R *accU[4];         // alpha terms, reference order
+	double alpha[4];
+	int nAlpha;
+	const R *accD[3];         // stored derivatives of earlier stages (beta terms)
+	double beta[3];
+	int nStoredD;
+	R *storeD;                // store this stage's derivative (if later stages use it)
+	double betaNew;           // beta of this stage's derivative (0: not used)
+	int n[3];
+	long ps;                  // plane stride (reals)
+	long step[3];             // 1, nx, nx*ny
+	int bc[6];
+	R fixedv[6][HYDRO_MAX_STATES];   // HYDRO_BC_FIXED ghost values (integrated then static fields)
+	R areaOverVol[3];         // areaL * (1 / volume) per side (fvsolver.cl:64-102)
+	int derivOn;              // volume > 1e-7 (fvsolver.cl:97)
+	int fluxOn[3];            // cell_area > 1e-7 (fvsolver.lua:105)
+	R dx[3];
+	double dxD[3];
+	// dt: device slot (cfl * dtmin[slot]) or a host value
+	DevScalars *dev;
+	int dtSlot;               // -1: use hostDt
+	double hostDt;
+	double cfl;
+	int resetSlot;            // slot to clear to +inf (block 0), -1 none
+	int reduceSlot;           // slot to min-reduce the new state's dt into, -1 none
+	int advanceTime;          // block 0: t += dt, dt stored
+	// equation parameters in the stencil type and in double (for the FP64
+	// accumulation, constrainU and dt work of the FP32 form)
+	R gamma, rhoMin, PMin, divPhi, divPsi;
+	double gammaD, rhoMinD, PMinD, divPhiD, divPsiD;
+	int limiter;              // flux or slope limiter index (runtime switch form)
+	int stripsX, stripsY, rowsPerWave;
+	int nBlocks;
+	int xcdRemap;
+};
+
+// ---------------------------------------------------------------- lane moves
+// Lane l receives lane l-1's value (wave_shr:1) / lane l+1's (wave_shl:1).
+__device__ __forceinline__ int dpp_from_left_i(int v) {
+	return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dpp_from_right_i(int v) {
+	return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false);
+}
+__device__ __forceinline__ double from_left(double v) {
+	int2 p = __builtin_bit_cast(int2, v);
+	p.x = dpp_from_left_i(p.x);
+	p.y = dpp_from_left_i(p.y);
+	return __builtin_bit_cast(double, p);
+}
+__device__ __forceinline__ double from_right(double v) {
+	int2 p = __builtin_bit_cast(int2, v);
+	p.x = dpp_from_right_i(p.x);
+	p.y = dpp_from_right_i(p.y);
+	return __builtin_bit_cast(double, p);
+}
+__device__ __forceinline__ float from_left(float v) {
+	return __builtin_bit_cast(float, dpp_from_left_i(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float from_right(float v) {
+	return __builtin_bit_cast(float, dpp_from_right_i(__builtin_bit_cast(int, v)));
+}
+
+// ------------------------------------------------------- boundary mapping
+// A ghost coordinate maps to an interior source cell, a sign for mirror
+// reflection, or a fixed value (gridsolver.lua:630-780). The reference fills
+// ghosts axis by axis (x, then y, then z), so a later axis wins at corners:
+// callers resolve z first, then y, then x.
+struct Mapped {
+	int idx;      // interior coordinate
+	int mirror;   // 1 if the normal vector component flips
+	int fixedFace;// face index when a fixed value applies, else -1
+};
+
+__device__ __forceinline__ Mapped map_coord(int c, int n, int kmin, int kmax, int faceMin) {
+	Mapped m;
+	m.idx = c;
+	m.mirror = 0;
+	m.fixedFace = -1;
+	if (c < 0) {
+		switch (kmin) {
+		case HYDRO_BC_PERIODIC: m.idx = ((c % n) + n) % n; break;
+		case HYDRO_BC_MIRROR: m.idx = -1 - c; m.mirror = 1; break;
+		case HYDRO_BC_FREEFLOW: m.idx = 0; break;
+		default: m.idx = 0; m.fixedFace = faceMin; break;
+		}
+	} else if (c >= n) {
+		switch (kmax) {
+		case HYDRO_BC_PERIODIC: m.idx = c % n; break;
+		case HYDRO_BC_MIRROR: m.idx = 2 * n - 1 - c; m.mirror = 1; break;
+		case HYDRO_BC_FREEFLOW: m.idx = n - 1; break;
+		default: m.idx = n - 1; m.fixedFace = faceMin + 1; break;
+		}
+	}
+	// guard tiny grids (n < ghost width): keep every access in bounds
+	if (m.idx < 0) m.idx = 0;
+	if (m.idx >= n) m.idx = n - 1;
+	return m;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+	for (int off = 32; off >= 1; off >>= 1) {
+		double o = __shfl_xor(v, off, 64);
+		v = o < v ? o : v;
+	}
+	return v;
+}
+
+__device__ __forceinline__ void atomic_min_pos(double *p, double v) {
+	// positive doubles (and +inf) order like their bit patterns
+	atomicMin(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+// Logical block id with blocks spread so that each XCD (blocks are dealt
+// round-robin over the 8 XCDs) gets a contiguous range of logical ids, so
+// neighbouring strips share an L2. A bijection for any block count.
+__device__ __forceinline__ int xcd_block(int b, int nBlocks) {
+	int q = nBlocks >> 3, r = nBlocks & 7;
+	int x = b & 7;
+	int start = x * q + (x < r ? x : r);
+	return start + (b >> 3);
+}
+
+} // namespace hydro

@@ -1,0 +1,11 @@
+// FP64 kernels. Built with -ffp-contract=off so the arithmetic is the
+// reference's expression order, operation for operation.
+#define REAL double
+#include "fv_kernels.inc"
+
+namespace hydro {
+hipError_t launch_reset_slots(DevScalars *dev, int mask, hipStream_t s) {
+	hipLaunchKernelGGL(reset_slots_kernel, dim3(1), dim3(64), 0, s, dev, mask);
+	return hipGetLastError();
+}
+} // namespace hydro

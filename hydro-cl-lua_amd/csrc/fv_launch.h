@@ -1,0 +1,50 @@
+// fv_launch.h — host-side launchers of the device kernels, one set per stored
+// precision (defined in fv_f64.hip and fv_f32.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fv_common.h"
+
+namespace hydro {
+
+struct KSum;
+
+// Selects one instantiation of the stage kernel.
+struct StageKey {
+	int eqn;        // HYDRO_EQN_*
+	int dim;        // 1, 2
+	int plm;        // plm cons on
+	int lim;        // Roe flux limiter on
+	int limiter;    // limiter index (flux limiter if lim, slope limiter if plm)
+};
+
+// Limiters with a compile-time form; the rest use the runtime switch.
+inline int limiter_template_id(int index) {
+	switch (index) {
+	case HYDRO_LIMITER_MINMOD:
+	case HYDRO_LIMITER_SUPERBEE:
+	case HYDRO_LIMITER_VAN_LEER:
+	case HYDRO_LIMITER_MONOTIZED_CENTRAL:
+		return index;
+	}
+	return -1;
+}
+
+template <typename R> struct GridArgs;
+
+template <typename R>
+hipError_t launch_stage(const StageKey &k, const StageArgs<R> &a, dim3 grid, dim3 block, hipStream_t s);
+
+template <typename R>
+hipError_t launch_dt(int eqn, int dim, const GridArgs<R> &g, hipStream_t s);
+
+template <typename R>
+hipError_t launch_constrain(int eqn, const GridArgs<R> &g, hipStream_t s);
+
+template <typename R>
+hipError_t launch_totals(const R *U, long ps, long ncell, int q, KSum *partial, int nPartial, double *out, hipStream_t s);
+
+hipError_t launch_reset_slots(DevScalars *dev, int mask, hipStream_t s);
+
+} // namespace hydro

@@ -1,0 +1,51 @@
+// fv_limiters.h — flux / slope limiter table of hydro/app.lua:614-636.
+//
+// LIM >= 0 selects one limiter at compile time; LIM == -1 dispatches on the
+// runtime index (uniform across the wave). van Leer uses
+// `r > 0 ? r * 2 / (1 + r) : 0`, equal to the reference's
+// `max(0., r) * 2. / (1. + r)` except at r = -1, where the reference divides
+// 0 by 0.
+#pragma once
+
+#include "fv_common.h"
+
+namespace hydro {
+
+template <typename R>
+__device__ __forceinline__ R limiter_rt(int index, R r) {
+	const R z = R(0), one = R(1), two = R(2);
+	switch (index) {
+	case 0: return z;
+	case 1: return one;
+	case 2: return r;
+	case 3: return R(.5) * (one + r);
+	case 4: return fmax(z, r) * (R(3) * r + one) / ((r + one) * (r + one));
+	case 5: return fmax(z, R(1.5) * (r + fabs(r)) / (r + two));
+	case 6: return fmax(z, two * (r + fabs(r)) / (r + R(3)));
+	case 7: return fmax(z, fmin(two * r, fmin((one + two * r) / R(3), two)));
+	case 8: return fmax(z, fmin(r, one));
+	case 9: return fmax(z, fmin(r, R(1.5)));
+	case 10: return R(.5) * (r * r + r) / (r * r + r + one);
+	case 11: return fmax(z, fmin(two * r, fmin(R(.25) + R(.75) * r, R(4))));
+	case 12: return fmax(z, fmax(fmin(R(1.5) * r, one), fmin(r, R(1.5))));
+	case 13: return fmax(z, fmin(fmin(two * r, R(.75) + R(.25) * r), fmin(R(.25) + R(.75) * r, two)));
+	case 14: return (r * r + r) / (r * r + one);
+	case 15: return two * r / (r * r + one);
+	case 16: return r > z ? r * two / (one + r) : z;
+	case 17: return fmax(z, fmin(two, fmin(R(.5) * (one + r), two * r)));
+	case 18: return fmax(z, fmax(fmin(one, two * r), fmin(two, r)));
+	case 19: return R(.5) * (r + one) * fmin(one, fmin(R(4) * r / (r + one), R(4) / (r + one)));
+	}
+	return z;
+}
+
+template <int LIM, typename R>
+__device__ __forceinline__ R limiter(int index, R r) {
+	if constexpr (LIM == HYDRO_LIMITER_MINMOD) return fmax(R(0), fmin(r, R(1)));
+	else if constexpr (LIM == HYDRO_LIMITER_SUPERBEE) return fmax(R(0), fmax(fmin(R(1), R(2) * r), fmin(R(2), r)));
+	else if constexpr (LIM == HYDRO_LIMITER_VAN_LEER) return r > R(0) ? r * R(2) / (R(1) + r) : R(0);
+	else if constexpr (LIM == HYDRO_LIMITER_MONOTIZED_CENTRAL) return fmax(R(0), fmin(R(2), fmin(R(.5) * (R(1) + r), R(2) * r)));
+	else return limiter_rt<R>(index, r);
+}
+
+} // namespace hydro

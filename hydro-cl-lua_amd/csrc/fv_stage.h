@@ -1,0 +1,354 @@
+// fv_stage.h — the fused finite-volume stage kernel (1D / 2D).
+//
+// One launch performs one integrator stage of the reference's
+//   calcLR -> calcFlux -> calcDerivFromFlux -> multAdd... -> boundary -> constrainU
+// sequence (fvsolver.lua:225-302, rk.lua:152-183) for every interior cell, and
+// on the last stage also the per-cell CFL dt of the new state and its
+// min-reduction (calcDT.cl:38-73, solverbase.lua:3004-3023).
+//
+// Work decomposition: one 64-lane wave owns a strip of 64 columns (60 output
+// columns plus a 2-cell halo each side) and marches down the strip in y.
+// x neighbours are exchanged with DPP wave shifts; y neighbours stay in
+// registers (a sliding window of rows), so every state value is loaded once
+// per stage from HBM/L2 and no LDS or barriers are needed. Ghost cells are not
+// stored: loads outside the interior are mapped through the boundary
+// conditions.
+#pragma once
+
+#include "fv_common.h"
+#include "fv_euler.h"
+#include "fv_limiters.h"
+
+namespace hydro {
+
+template <typename R, class E>
+struct Cell {
+	static constexpr int NV = E::Q + E::NSTAT;
+	R v[NV];
+};
+
+// ---------------------------------------------------------- state loading
+template <typename R, class E>
+__device__ __forceinline__ void load_cell(const StageArgs<R> &a, const R *base, const R *stat,
+	const Mapped &mx, const Mapped &my, Cell<R, E> &c)
+{
+	long off = (long)my.idx * a.step[1] + mx.idx;
+#pragma unroll
+	for (int f = 0; f < Cell<R, E>::NV; ++f) {
+		R v;
+		if (f < E::Q) v = base[f * a.ps + off];
+		else v = stat[(f - E::Q) * a.ps + off];
+		if (mx.fixedFace >= 0) v = a.fixedv[mx.fixedFace][f];
+		else if (mx.mirror && E::reflects(f, 0)) v = -v;
+		if (my.fixedFace >= 0) v = a.fixedv[my.fixedFace][f];
+		else if (my.mirror && E::reflects(f, 1)) v = -v;
+		c.v[f] = v;
+	}
+}
+
+template <typename R, class E>
+__device__ __forceinline__ void shift_left(const Cell<R, E> &in, Cell<R, E> &out) {
+#pragma unroll
+	for (int f = 0; f < Cell<R, E>::NV; ++f) out.v[f] = from_left(in.v[f]);
+}
+template <typename R, class E>
+__device__ __forceinline__ void shift_right(const Cell<R, E> &in, Cell<R, E> &out) {
+#pragma unroll
+	for (int f = 0; f < Cell<R, E>::NV; ++f) out.v[f] = from_right(in.v[f]);
+}
+
+// ------------------------------------------------------------- plm cons
+// plm.cl:27-91: one division per field (numerator / denominator chosen by the
+// sign of dUC, as the reference's two branches do)
+template <typename R, class E, int SLIM>
+__device__ __forceinline__ void plm_cons(int lim, const Cell<R, E> &Um, const Cell<R, E> &U,
+	const Cell<R, E> &Up, Cell<R, E> &L, Cell<R, E> &Rr)
+{
+#pragma unroll
+	for (int j = 0; j < Cell<R, E>::NV; ++j) {
+		if (j < E::Q) {
+			R dUR = Up.v[j] - U.v[j];
+			R dUL = U.v[j] - Um.v[j];
+			R dUC = R(.5) * (dUR - dUL);
+			bool pos = dUC >= R(0);
+			R num = pos ? dUL : dUR;
+			R den = pos ? dUR : dUL;
+			R r = den == R(0) ? R(0) : (num / den);
+			R sigma = limiter<SLIM>(lim, r) * den;
+			Rr.v[j] = U.v[j] + R(.5) * sigma;
+			L.v[j] = U.v[j] - R(.5) * sigma;
+		} else {
+			L.v[j] = U.v[j];
+			Rr.v[j] = U.v[j];
+		}
+	}
+}
+
+// ------------------------------------------------------------------ Roe
+// Interface quantities shared by neighbouring interfaces when the flux
+// limiter is on: the Roe state, L * (UR - UL), and the mean physical flux.
+template <typename R, class E>
+struct Iface {
+	typename E::Eig e;
+	R dW[E::NW];
+	R lambda[E::NW];
+	R Favg[E::Q];
+};
+
+template <typename R, class E, int S>
+__device__ __forceinline__ void iface_prep(const typename E::Par &p, const Cell<R, E> &UL,
+	const Cell<R, E> &UR, Iface<R, E> &I)
+{
+	auto WL = E::prim(p, UL.v);
+	auto WR = E::prim(p, UR.v);
+	I.e = E::eigen(p, UL.v, UR.v, WL, WR);
+	E::template waves<S>(I.e, I.lambda);
+	R dU[E::Q];
+#pragma unroll
+	for (int j = 0; j < E::Q; ++j) dU[j] = UR.v[j] - UL.v[j];
+	E::template left<S>(p, I.e, dU, I.dW);
+	R FL[E::Q], FR[E::Q];
+	E::template flux<S>(UL.v, WL, FL);
+	E::template flux<S>(UR.v, WR, FR);
+#pragma unroll
+	for (int j = 0; j < E::Q; ++j) I.Favg[j] = R(.5) * (FL[j] + FR[j]);
+}
+
+// roe.cl:91-160
+template <typename R, class E, int S, bool LIM, int FLIM>
+__device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface<R, E> &I,
+	const R *dWL, const R *dWR, R dt_dx, int lim, R *F)
+{
+	R fluxEig[E::NW];
+#pragma unroll
+	for (int j = 0; j < E::NW; ++j) {
+		R lambda = I.lambda[j];
+		R sgnLambda = lambda >= R(0) ? R(1) : R(-1);
+		if constexpr (LIM) {
+			R rEig;
+			if (I.dW[j] == R(0)) rEig = R(0);
+			else rEig = lambda >= R(0) ? dWL[j] / I.dW[j] : dWR[j] / I.dW[j];
+			R phi = limiter<FLIM>(lim, rEig);
+			fluxEig[j] = R(0) - R(.5) * lambda * I.dW[j] * (sgnLambda + phi * (lambda * dt_dx - sgnLambda));
+		} else {
+			fluxEig[j] = R(0) - R(.5) * lambda * I.dW[j] * sgnLambda;
+		}
+	}
+	R Y[E::Q];
+	E::template right<S>(p, I.e, fluxEig, Y);
+#pragma unroll
+	for (int j = 0; j < E::Q; ++j) F[j] = Y[j] + I.Favg[j];
+}
+
+// ---------------------------------------------------------- the kernel
+template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
+__global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
+{
+	using E = EQ<R>;
+	using C = Cell<R, E>;
+	constexpr int Q = E::Q;
+	const typename E::Par par = E::par(a);
+	const int lim = PLM ? 0 : a.limiter;   // runtime limiter index when FLIM/SLIM == -1
+	const int slim = a.limiter;
+
+	const int lane = threadIdx.x & 63;
+	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
+	const int wave = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+
+	// dt of this step (solverbase.lua:3016): cfl * min, or a host value
+	double dt = a.dtSlot >= 0 ? a.cfl * a.dev->dtmin[a.dtSlot] : a.hostDt;
+	if (blockIdx.x == 0 && threadIdx.x == 0 && a.resetSlot >= 0) a.dev->dtmin[a.resetSlot] = INFINITY;
+
+	const int nx = a.n[0];
+	const int ny = DIM >= 2 ? a.n[1] : 1;
+	const int sx = wave % a.stripsX;
+	const int sy = wave / a.stripsX;
+	double dtLocal = INFINITY;
+
+	if (sy < a.stripsY) {
+		const int x = sx * STRIP_OUT - NG + lane;
+		const bool outLane = lane >= NG && lane < WAVE - NG && x < nx;
+		const Mapped mx = map_coord(x, nx, a.bc[0], a.bc[1], 0);
+		Mapped myIn;
+		myIn.idx = 0; myIn.mirror = 0; myIn.fixedFace = -1;
+		const R rdt = R(dt);
+		const R dt_dx0 = rdt / a.dx[0];
+		const R dt_dx1 = DIM >= 2 ? rdt / a.dx[1] : R(0);
+		const int y0 = sy * a.rowsPerWave;
+		const int y1 = min(y0 + a.rowsPerWave, ny);
+
+		// y pipeline state
+		C W0, W1, W2;
+		C Ry0;                  // PLM: R edge of row r (from the previous iteration)
+		Iface<R, E> I0, I1;     // LIM: interfaces r-1/2 and r+1/2
+		R FaYprev[Q];
+#pragma unroll
+		for (int f = 0; f < C::NV; ++f) { W0.v[f] = W1.v[f] = W2.v[f] = Ry0.v[f] = R(0); }
+#pragma unroll
+		for (int j = 0; j < Q; ++j) FaYprev[j] = R(0);
+		I0 = Iface<R, E>{};
+		I1 = Iface<R, E>{};
+
+		auto loadRow = [&](int r, C &c) {
+			Mapped my = DIM >= 2 ? map_coord(r, ny, a.bc[2], a.bc[3], 2) : myIn;
+			load_cell<R, E>(a, a.src, a.stat, mx, my, c);
+		};
+
+		// iterations before y0 only fill the pipeline
+		constexpr int WARM = DIM < 2 ? 0 : (PLM ? 2 : (LIM ? 3 : 1));
+		if constexpr (DIM >= 2) {
+			if constexpr (PLM) { loadRow(y0 - 2, W0); loadRow(y0 - 1, W1); }
+			else if constexpr (LIM) { loadRow(y0 - 2, W1); }
+			else { loadRow(y0 - 1, W0); }
+		}
+
+		for (int r = y0 - WARM; r < y1; ++r) {
+			// ---- y direction: W0 = U(r) after this block
+			R dyterm[Q];
+			if constexpr (DIM >= 2) {
+				R FaY[Q];
+				R F[Q];
+				if constexpr (PLM) {
+					// W0 = U(r), W1 = U(r+1): cell r+1's edges, interface r+1/2
+					loadRow(r + 2, W2);
+					C L1, R1;
+					plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
+					Iface<R, E> I;
+					iface_prep<R, E, 1>(par, Ry0, L1, I);
+					iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, F);
+					Ry0 = R1;
+				} else if constexpr (LIM) {
+					// W1 = U(r+1): interface r+3/2, flux at r+1/2 from its neighbours
+					loadRow(r + 2, W2);
+					Iface<R, E> I2;
+					iface_prep<R, E, 1>(par, W1, W2, I2);
+					iface_flux<R, E, 1, true, FLIM>(par, I1, I0.dW, I2.dW, dt_dx1, lim, F);
+					I0 = I1;
+					I1 = I2;
+				} else {
+					// W0 = U(r): interface r+1/2
+					loadRow(r + 1, W1);
+					Iface<R, E> I;
+					iface_prep<R, E, 1>(par, W0, W1, I);
+					iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, F);
+				}
+#pragma unroll
+				for (int j = 0; j < Q; ++j) {
+					FaY[j] = a.fluxOn[1] ? F[j] * a.areaOverVol[1] : R(0);
+					dyterm[j] = FaY[j] - FaYprev[j];
+					FaYprev[j] = FaY[j];
+				}
+			}
+
+			if (r >= y0) {
+				// ---- x direction on row r (lanes)
+				C U;
+				if constexpr (DIM >= 2) U = W0;
+				else loadRow(0, U);
+				R Fx[Q];
+				if constexpr (PLM) {
+					C Um, Up, Lx, Rx, RxL;
+					shift_left<R, E>(U, Um);
+					shift_right<R, E>(U, Up);
+					plm_cons<R, E, SLIM>(slim, Um, U, Up, Lx, Rx);
+					shift_left<R, E>(Rx, RxL);
+					Iface<R, E> I;
+					iface_prep<R, E, 0>(par, RxL, Lx, I);
+					iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+				} else if constexpr (LIM) {
+					C Um;
+					shift_left<R, E>(U, Um);
+					Iface<R, E> I;
+					iface_prep<R, E, 0>(par, Um, U, I);
+					R dWL[E::NW], dWR[E::NW];
+#pragma unroll
+					for (int j = 0; j < E::NW; ++j) { dWL[j] = from_left(I.dW[j]); dWR[j] = from_right(I.dW[j]); }
+					iface_flux<R, E, 0, true, FLIM>(par, I, dWL, dWR, dt_dx0, lim, Fx);
+				} else {
+					C Um;
+					shift_left<R, E>(U, Um);
+					Iface<R, E> I;
+					iface_prep<R, E, 0>(par, Um, U, I);
+					iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+				}
+				R deriv[Q];
+#pragma unroll
+				for (int j = 0; j < Q; ++j) {
+					R Fa = a.fluxOn[0] ? Fx[j] * a.areaOverVol[0] : R(0);
+					R FaR = from_right(Fa);
+					deriv[j] = R(0) - (FaR - Fa);
+					if constexpr (DIM >= 2) deriv[j] = deriv[j] - dyterm[j];
+					if (!a.derivOn) deriv[j] = R(0);
+				}
+
+				// ---- integrator combine (rk.lua:155-174, fe.lua:50), constrainU
+				if (outLane) {
+					long off = (long)r * a.step[1] + x;
+					double acc[Q];
+#pragma unroll
+					for (int j = 0; j < Q; ++j) acc[j] = 0.;
+					for (int k = 0; k < a.nAlpha; ++k) {
+						const R *Uk = a.accU[k];
+						double al = a.alpha[k];
+						bool fromSrc = Uk == a.src;
+#pragma unroll
+						for (int j = 0; j < Q; ++j) {
+							R u = fromSrc ? U.v[j] : Uk[j * a.ps + off];
+							acc[j] = acc[j] + (double)u * al;
+						}
+					}
+					for (int k = 0; k < a.nStoredD; ++k) {
+						const R *Dk = a.accD[k];
+						double c = (double)R(a.beta[k] * dt);
+#pragma unroll
+						for (int j = 0; j < Q; ++j) acc[j] = acc[j] + (double)Dk[j * a.ps + off] * c;
+					}
+					if (a.storeD) {
+#pragma unroll
+						for (int j = 0; j < Q; ++j) a.storeD[j * a.ps + off] = deriv[j];
+					}
+					if (a.betaNew != 0.) {
+						double c = (double)R(a.betaNew * dt);
+#pragma unroll
+						for (int j = 0; j < Q; ++j) acc[j] = acc[j] + (double)deriv[j] * c;
+					}
+					E::template constrain<double>(a.gammaD, a.rhoMinD, a.PMinD, acc);
+					R o[Q];
+#pragma unroll
+					for (int j = 0; j < Q; ++j) {
+						o[j] = R(acc[j]);
+						a.out[j * a.ps + off] = o[j];
+					}
+					if (a.reduceSlot >= 0) {
+						double ud[C::NV];
+						const double *dxd = a.dxD;
+#pragma unroll
+						for (int j = 0; j < Q; ++j) ud[j] = (double)o[j];
+#pragma unroll
+						for (int j = Q; j < C::NV; ++j) ud[j] = (double)U.v[j];
+						double d = E::template dt_cell<double, DIM>(a.gammaD, a.rhoMinD,
+							a.PMinD, dxd, ud, a.divPhiD, a.divPsiD);
+						dtLocal = fmin(dtLocal, d);
+					}
+				}
+			}
+
+			// ---- advance the row window
+			if constexpr (DIM >= 2) {
+				if constexpr (PLM || LIM) { W0 = W1; W1 = W2; }
+				else { W0 = W1; }
+			}
+		}
+	}
+
+	if (a.reduceSlot >= 0) {
+		double m = wave_min(dtLocal);
+		if (lane == 0 && m < INFINITY) atomic_min_pos(&a.dev->dtmin[a.reduceSlot], m);
+	}
+	if (a.advanceTime && blockIdx.x == 0 && threadIdx.x == 0) {
+		a.dev->t = a.dev->t + dt;
+		a.dev->dt = dt;
+	}
+}
+
+} // namespace hydro

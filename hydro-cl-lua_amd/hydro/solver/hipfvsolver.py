@@ -1,0 +1,164 @@
+"""HipFVSolver: the reference's FiniteVolumeSolver interface over the HIP C ABI.
+
+Mirrors `require 'hydro.solver.fvsolver'(args)` (hydro/solver/fvsolver.lua,
+gridsolver.lua, solverbase.lua): the same construction arguments (dim,
+gridSize, mins, maxs, eqn, flux, fluxLimiter, usePLM, slopeLimiter,
+integrator, cfl, fixedDT, boundary, initCond) and the methods the app calls —
+update, step(dt), calcDT, boundary, resetState, constrainU — plus
+UBufObj:toCPU()/fromCPU() on the ghosted cons_t buffer, and t / dt.
+Errors raise like the reference's asserts.
+"""
+import ctypes
+
+import numpy as np
+
+from .. import _abi
+from .._cfg import make_cfg, cfg_ghost_shape, INTEGRATOR_STAGES
+from ..init import euler as init_euler
+
+
+class UBufObj:
+    """`solver.UBufObj`: the cons_t buffer with ghost cells (x fastest)."""
+
+    def __init__(self, solver):
+        self.solver = solver
+
+    def toCPU(self):
+        s = self.solver
+        out = np.empty(s.shape, dtype=s.dtype)
+        _abi.check(_abi.lib().hydro_download_U(s.h, out.ctypes.data, out.nbytes))
+        return out
+
+    def fromCPU(self, U):
+        s = self.solver
+        U = np.ascontiguousarray(np.asarray(U, dtype=s.dtype).reshape(s.shape))
+        _abi.check(_abi.lib().hydro_upload_U(s.h, U.ctypes.data, U.nbytes))
+
+
+_INIT_CONDS = {
+    'Sod': lambda cfg, gamma, args: init_euler.sod(cfg, gamma),
+    'Kelvin-Helmholtz': lambda cfg, gamma, args: init_euler.kelvin_helmholtz(cfg, gamma, **args),
+    'sphere': lambda cfg, gamma, args: init_euler.sphere_blast(cfg, gamma, **args),
+}
+
+
+class HipFVSolver:
+    name = 'hipfvsolver'
+    numGhost = 2
+
+    def __init__(self, dim, gridSize, eqn='euler', flux='roe', integrator='forward Euler',
+                 fluxLimiter='donor cell', usePLM=None, slopeLimiter='minmod', cfl=.5,
+                 fixedDT=None, mins=None, maxs=None, boundary='freeflow', boundaryFixed=None,
+                 initCond=None, initCondArgs=None, real='double', eqnArgs=None):
+        eqnArgs = dict(eqnArgs or {})
+        self.cfg = make_cfg(dim=dim, size=gridSize, mins=mins, maxs=maxs, eqn=eqn, flux=flux,
+                            flux_limiter=fluxLimiter, plm=usePLM, slope_limiter=slopeLimiter,
+                            integrator=integrator, cfl=cfl, fixed_dt=fixedDT, boundary=boundary,
+                            boundary_fixed=boundaryFixed, precision=real,
+                            gamma=eqnArgs.get('heatCapacityRatio', 7 / 5),
+                            rho_min=eqnArgs.get('rhoMin', 1e-7), P_min=eqnArgs.get('PMin', 1e-7),
+                            div_phi_wavespeed=eqnArgs.get('divPhiWavespeed', 1.),
+                            div_psi_wavespeed=eqnArgs.get('divPsiWavespeed', 1.))
+        self.dim = int(dim)
+        self.dtype = np.float64 if real == 'double' else np.float32
+        self.shape = cfg_ghost_shape(self.cfg)
+        self.initCond = initCond
+        self.initCondArgs = dict(initCondArgs or {})
+        self.cfl = float(cfl)
+        self.stages = INTEGRATOR_STAGES[self.cfg.integrator]
+        h = ctypes.c_void_p()
+        _abi.check(_abi.lib().hydro_create(ctypes.byref(self.cfg), ctypes.byref(h)))
+        self.h = h
+        self.UBufObj = UBufObj(self)
+        if initCond is not None:
+            self.resetState()
+
+    def __del__(self):
+        h = getattr(self, 'h', None)
+        if h:
+            _abi.lib().hydro_destroy(h)
+            self.h = None
+
+    # -- sizes (gridsolver.lua:60-96)
+    @property
+    def numCells(self):
+        return int(_abi.lib().hydro_num_cells(self.h))
+
+    @property
+    def gridSize(self):
+        return tuple(self.cfg.size[i] + 4 if i < self.dim else 1 for i in range(3))
+
+    @property
+    def sizeWithoutBorder(self):
+        return tuple(self.cfg.size[i] if i < self.dim else 1 for i in range(3))
+
+    @property
+    def deviceBytes(self):
+        return int(_abi.lib().hydro_device_bytes(self.h))
+
+    # -- time
+    @property
+    def t(self):
+        t = ctypes.c_double()
+        _abi.check(_abi.lib().hydro_get_time(self.h, ctypes.byref(t), None))
+        return t.value
+
+    @t.setter
+    def t(self, v):
+        _abi.check(_abi.lib().hydro_set_time(self.h, float(v)))
+
+    @property
+    def dt(self):
+        dt = ctypes.c_double()
+        _abi.check(_abi.lib().hydro_get_time(self.h, None, ctypes.byref(dt)))
+        return dt.value
+
+    # -- the reference's solver methods
+    def resetState(self, U=None):
+        """SolverBase:resetState (solverbase.lua:2094-2114): initial state, boundary, constrainU"""
+        if U is None:
+            if self.initCond not in _INIT_CONDS:
+                raise ValueError(f'unknown initCond {self.initCond!r}')
+            Uint = _INIT_CONDS[self.initCond](self.cfg, self.cfg.gamma, self.initCondArgs)
+            U = init_euler.to_ghosted(self.cfg, Uint)
+        self.UBufObj.fromCPU(U)
+        self.t = 0.
+        self.constrainU()
+
+    def boundary(self):
+        _abi.check(_abi.lib().hydro_boundary(self.h))
+
+    def constrainU(self):
+        _abi.check(_abi.lib().hydro_constrain_U(self.h))
+
+    def calcDT(self):
+        dt = ctypes.c_double()
+        _abi.check(_abi.lib().hydro_calc_dt(self.h, ctypes.byref(dt)))
+        return dt.value
+
+    def step(self, dt):
+        _abi.check(_abi.lib().hydro_step(self.h, float(dt)))
+
+    def update(self, n=1):
+        """n x SolverBase:update (calcDT, step, t += dt) with dt kept on the device"""
+        _abi.check(_abi.lib().hydro_update(self.h, int(n)))
+
+    def sync(self):
+        _abi.check(_abi.lib().hydro_sync(self.h))
+
+    def conservedTotals(self):
+        q = 5 if self.cfg.eqn == 0 else 8
+        out = (ctypes.c_double * q)()
+        _abi.check(_abi.lib().hydro_conserved_totals(self.h, out))
+        return np.array(out[:])
+
+    def timeUpdates(self, n, perKernel=True):
+        t = _abi.HydroTiming()
+        _abi.check(_abi.lib().hydro_time_updates(self.h, int(n), 1 if perKernel else 0, ctypes.byref(t)))
+        return t
+
+    def interior(self, U):
+        sl = [slice(None)] * 4
+        for i in range(self.dim):
+            sl[2 - i] = slice(2, -2)
+        return U[tuple(sl)]

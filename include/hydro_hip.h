@@ -1,0 +1,206 @@
+/*
+ * hydro_hip.h — C ABI of the MI355X finite-volume hot path.
+ *
+ * Plain C: opaque handles, int status codes (0 = ok, <0 = error, message in
+ * hydro_last_error()), caller-owned host arrays, library-owned device memory,
+ * one HIP stream per handle. A handle is thread-compatible, not thread-safe.
+ * The same header is meant for LuaJIT `ffi.cdef` + `ffi.load` (the reference
+ * binds C the same way, hydro/code/safecdef.lua:1-16) and for Python ctypes.
+ *
+ * Each entry point names the reference interface it replaces (file:line,
+ * relative to the hydro-cl-lua tree).
+ */
+#ifndef HYDRO_HIP_H
+#define HYDRO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Largest cons_t of the supported equations (GLM-Maxwell: 12 reals). */
+#define HYDRO_MAX_STATES 12
+
+/* Equation plugins: hydro/eqn/euler.lua (cons_t = rho, m.xyz, ETotal, ePot;
+ * 5 integrated, 5 waves) and hydro/eqn/glm-maxwell.lua (cons_t = D.xyz, B.xyz,
+ * phi, psi, rhoCharge, sigma, sqrt_1_eps, sqrt_1_mu; 8 integrated, 8 waves). */
+enum { HYDRO_EQN_EULER = 0, HYDRO_EQN_GLM_MAXWELL = 1 };
+
+/* Flux plugins: hydro/flux/roe.lua. */
+enum { HYDRO_FLUX_ROE = 0 };
+
+/* Limiter table in the order of HydroCLApp.limiters (hydro/app.lua:614-636),
+ * 0-based: the reference's index 1 ('donor cell') is 0 here, and a flux limiter
+ * is active when flux_limiter != HYDRO_LIMITER_DONOR_CELL (the reference's
+ * `solver.fluxLimiter > 1`, hydro/flux/roe.cl:12). */
+enum {
+	HYDRO_LIMITER_DONOR_CELL = 0,
+	HYDRO_LIMITER_LAX_WENDROFF,
+	HYDRO_LIMITER_BEAM_WARMING,
+	HYDRO_LIMITER_FROMM,
+	HYDRO_LIMITER_CHARM,
+	HYDRO_LIMITER_HCUS,
+	HYDRO_LIMITER_HQUICK,
+	HYDRO_LIMITER_KOREN,
+	HYDRO_LIMITER_MINMOD,
+	HYDRO_LIMITER_OSHKER,
+	HYDRO_LIMITER_OSPRE,
+	HYDRO_LIMITER_SMART,
+	HYDRO_LIMITER_SWEBY,
+	HYDRO_LIMITER_UMIST,
+	HYDRO_LIMITER_VAN_ALBADA_1,
+	HYDRO_LIMITER_VAN_ALBADA_2,
+	HYDRO_LIMITER_VAN_LEER,
+	HYDRO_LIMITER_MONOTIZED_CENTRAL,
+	HYDRO_LIMITER_SUPERBEE,
+	HYDRO_LIMITER_BARTH_JESPERSEN,
+	HYDRO_NUM_LIMITERS
+};
+
+/* Reconstruction (solver.usePLM): none, or 'plm cons' (hydro/solver/plm.cl:27-91). */
+enum { HYDRO_PLM_NONE = 0, HYDRO_PLM_CONS = 1 };
+
+/* Explicit integrators, hydro/int/fe.lua and the Shu-Osher tables of
+ * hydro/int/all.lua:10-160 (same names). Implicit ones are not provided. */
+enum {
+	HYDRO_INT_FORWARD_EULER = 0,
+	HYDRO_INT_RK2,
+	HYDRO_INT_RK2_HEUN,
+	HYDRO_INT_RK2_RALSTON,
+	HYDRO_INT_RK3,
+	HYDRO_INT_RK4,
+	HYDRO_INT_RK4_3_8THS,
+	HYDRO_INT_RK2_TVD,
+	HYDRO_INT_RK2_NON_TVD,
+	HYDRO_INT_RK3_TVD,
+	HYDRO_INT_RK4_TVD,
+	HYDRO_INT_RK4_NON_TVD,
+	HYDRO_NUM_INTEGRATORS
+};
+
+/* Grid boundaries (hydro/solver/gridsolver.lua:567-780). Faces are ordered
+ * xmin, xmax, ymin, ymax, zmin, zmax. FIXED writes boundary_fixed[face] (a full
+ * cons_t) into the ghost cells, the reference's user `fixedCode`. */
+enum { HYDRO_BC_PERIODIC = 0, HYDRO_BC_MIRROR = 1, HYDRO_BC_FREEFLOW = 2, HYDRO_BC_FIXED = 3 };
+
+/* Precision of the stored state. F64: everything in double. F32: state and
+ * stencil arithmetic in float; dt min-reduction, conserved-total sums and the
+ * RK stage accumulation in double. */
+enum { HYDRO_REAL_F64 = 0, HYDRO_REAL_F32 = 1 };
+
+/* Solver construction arguments: the subset of the reference's `args`
+ * (hydro/solver/solverbase.lua:327-338, 775-812; gridsolver.lua:54-136;
+ * fvsolver.lua:28-31) that the finite-volume path reads. */
+typedef struct hydro_cfg {
+	int dim;                 /* 1, 2, 3 */
+	int size[3];             /* interior cells per axis (args.gridSize); unused axes 1 */
+	double mins[3];          /* args.mins (default -1) */
+	double maxs[3];          /* args.maxs (default 1) */
+	int eqn;                 /* HYDRO_EQN_* */
+	int flux;                /* HYDRO_FLUX_* */
+	int flux_limiter;        /* HYDRO_LIMITER_*, args.fluxLimiter */
+	int plm;                 /* HYDRO_PLM_*, args.usePLM */
+	int slope_limiter;       /* HYDRO_LIMITER_*, args.slopeLimiter */
+	int integrator;          /* HYDRO_INT_* */
+	double cfl;              /* args.cfl */
+	int use_fixed_dt;        /* args.fixedDT given */
+	double fixed_dt;
+	int boundary[6];         /* HYDRO_BC_* per face */
+	double boundary_fixed[6][HYDRO_MAX_STATES];
+	int precision;           /* HYDRO_REAL_* */
+	double gamma;            /* Euler heatCapacityRatio (euler.lua:194) */
+	double rho_min;          /* Euler rhoMin (euler.lua:195) */
+	double P_min;            /* Euler PMin (euler.lua:196) */
+	double div_phi_wavespeed;/* GLM-Maxwell divPhiWavespeed (glm-maxwell.lua:110) */
+	double div_psi_wavespeed;/* GLM-Maxwell divPsiWavespeed (glm-maxwell.lua:111) */
+} hydro_cfg;
+
+/* Reference defaults: mins -1, maxs 1 (gridsolver.lua:80-81), gamma 7/5,
+ * rhoMin = PMin = 1e-7 (euler.lua:192-197), cfl .5, boundaries freeflow
+ * (gridsolver.lua:124), limiters donor cell, Roe, forward Euler, F64. */
+void hydro_cfg_default(hydro_cfg *cfg);
+
+/* Last error message of the calling thread ("" if none). */
+const char *hydro_last_error(void);
+
+/* Number of reals in cons_t (numStates) and integrated (numIntStates). */
+int hydro_eqn_num_states(int eqn);
+int hydro_eqn_num_int_states(int eqn);
+
+/* Number of device GPUs visible (0 when none). No other call needs one. */
+int hydro_device_count(void);
+
+/* ---------------------------------------------------------------- plain solver */
+
+typedef struct hydro_solver hydro_solver;
+
+/* FiniteVolumeSolver(args) (fvsolver.lua, solverbase.lua:351-395): validates the
+ * configuration, allocates the device state. Errors like the reference's
+ * asserts (e.g. PLM together with a flux limiter, gridsolver.lua:119). */
+int hydro_create(const hydro_cfg *cfg, hydro_solver **out);
+int hydro_destroy(hydro_solver *h);
+
+/* Stored cells including numGhost=2 per used side (solver.numCells). */
+size_t hydro_num_cells(const hydro_solver *h);
+
+/* UBufObj:fromCPU / :toCPU. The host array is the reference's layout: cons_t
+ * (numStates reals of the solver's precision) per cell, x fastest, ghost cells
+ * included. Upload takes the interior; ghost cells of the download are filled
+ * from the boundary conditions, as after the reference's boundary(). */
+int hydro_upload_U(hydro_solver *h, const void *U, size_t bytes);
+int hydro_download_U(hydro_solver *h, void *U, size_t bytes);
+
+/* SolverBase:constrainU (solverbase.lua:2116-2127, euler.cl:697-717). */
+int hydro_constrain_U(hydro_solver *h);
+
+/* GridSolver:boundary (gridsolver.lua:1316). Ghost values are derived from the
+ * interior when read, so this only exists for call-compatibility. */
+int hydro_boundary(hydro_solver *h);
+
+/* SolverBase:calcDT (solverbase.lua:3004-3023): dt = cfl * min over interior
+ * cells (or fixedDT). Blocking: returns dt to the host. */
+int hydro_calc_dt(hydro_solver *h, double *dt);
+
+/* SolverBase:step(dt) (solverbase.lua:3193-3238): one integrator step with a
+ * host-given dt. Does not advance t. */
+int hydro_step(hydro_solver *h, double dt);
+
+/* nsteps x SolverBase:update (solverbase.lua:3026-3190): calcDT, step, t += dt,
+ * with dt kept on the device (no per-step readback). Asynchronous on the
+ * solver stream; hydro_get_time synchronizes. */
+int hydro_update(hydro_solver *h, int nsteps);
+
+/* solver.t and solver.dt after the last update (synchronizes). */
+int hydro_get_time(hydro_solver *h, double *t, double *dt);
+int hydro_set_time(hydro_solver *h, double t);
+
+/* Sum over interior cells of each integrated cons_t field, in double with
+ * compensated (Neumaier) summation and a fixed reduction order.
+ * out has numIntStates entries. */
+int hydro_conserved_totals(hydro_solver *h, double *out);
+
+/* Wait for the solver's stream. */
+int hydro_sync(hydro_solver *h);
+
+/* Device timing of nsteps updates with HIP events on the solver stream.
+ * total_ms: whole update loop; stage_ms: sum of the fused stage kernel
+ * durations (events around each launch); n_stage_launches: their count. */
+typedef struct hydro_timing {
+	double total_ms;
+	double stage_ms;
+	int n_stage_launches;
+	double dt_ms;            /* standalone dt kernels (first step only) */
+	int n_dt_launches;
+} hydro_timing;
+int hydro_time_updates(hydro_solver *h, int nsteps, int per_kernel_events, hydro_timing *out);
+
+/* Bytes of the device state held by the handle. */
+size_t hydro_device_bytes(const hydro_solver *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,78 @@
+/*
+ * hydro_oracle.h — TEST INFRASTRUCTURE. CPU restatement of hydro-cl-lua's
+ * structured-grid finite-volume path, used only by tests/, the smoke check and
+ * bench.py's cpu_baseline leg as the checker. Never linked into the product.
+ *
+ * FP64, the reference's layout (AoS cons_t with numGhost=2 ghost cells, x
+ * fastest) and the reference's kernel sequence with unfused buffers:
+ * calcLR -> calcFlux -> calcDerivFromFlux -> (addSource) -> multAdd ->
+ * boundary -> constrainU, and calcDT + min. OpenMP over cells.
+ */
+#ifndef HYDRO_ORACLE_H
+#define HYDRO_ORACLE_H
+
+#include <stddef.h>
+#include "../include/hydro_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_solver oracle_solver;
+
+/* equation parameters the per-cell functions read from solver_t */
+typedef struct oracle_params {
+	double gamma, rho_min, P_min, div_phi, div_psi;
+} oracle_params;
+
+oracle_solver *oracle_create(const hydro_cfg *cfg);
+void oracle_destroy(oracle_solver *s);
+const char *oracle_last_error(void);
+
+size_t oracle_num_cells(const oracle_solver *s);
+int oracle_num_states(const oracle_solver *s);
+double *oracle_U(oracle_solver *s);           /* AoS, ghosts included */
+void oracle_set_threads(int n);
+int oracle_get_threads(void);
+
+void oracle_boundary(oracle_solver *s);
+void oracle_constrain_U(oracle_solver *s);      /* constrainU kernel + boundary */
+double oracle_calc_dt(oracle_solver *s);        /* cfl * min (or fixed dt) */
+void oracle_step(oracle_solver *s, double dt);  /* integrator step */
+double oracle_update(oracle_solver *s);         /* calcDT, step, t += dt, boundary */
+double oracle_time(const oracle_solver *s);
+void oracle_set_time(oracle_solver *s, double t);
+/* calcDeriv into deriv (AoS, G*S); deriv is cleared first */
+void oracle_calc_deriv(oracle_solver *s, double dt, double *deriv);
+
+/* ---- per-interface / per-cell functions, for the pinning tests ---- */
+double oracle_limiter(int index, double r);
+/* Euler */
+void oracle_euler_prim_from_cons(const oracle_params *p, const double *U, double *W);
+void oracle_euler_cons_from_prim(const oracle_params *p, const double *W, double *U);
+void oracle_euler_eigen_for_interface(const oracle_params *p, const double *UL, const double *UR, double *eig);
+void oracle_euler_left(const oracle_params *p, const double *eig, const double *X, int side, double *Y);
+void oracle_euler_right(const oracle_params *p, const double *eig, const double *X, int side, double *Y);
+void oracle_euler_flux_from_cons(const oracle_params *p, const double *U, int side, double *F);
+void oracle_euler_waves(const oracle_params *p, const double *eig, int side, double *lambda);
+double oracle_euler_dt_cell(const oracle_params *p, const double *U, int dim, const double *dx);
+void oracle_euler_constrain(const oracle_params *p, double *U);
+/* GLM-Maxwell */
+void oracle_glm_eigen_for_interface(const oracle_params *p, const double *UL, const double *UR, double *eig);
+void oracle_glm_left(const oracle_params *p, const double *eig, const double *X, int side, double *Y);
+void oracle_glm_right(const oracle_params *p, const double *eig, const double *X, int side, double *Y);
+void oracle_glm_flux_from_cons(const oracle_params *p, const double *U, int side, double *F);
+void oracle_glm_waves(const oracle_params *p, const double *eig, int side, double *lambda);
+/* Roe flux for one interface (roe.cl:17-163). lim_args may be NULL when no
+ * flux limiter: otherwise {UL_L, UR_L, UL_R, UR_R} pointers. */
+void oracle_roe_flux(const oracle_params *p, int eqn, int flux_limiter, double dt_dx,
+	const double *UL, const double *UR, const double *const *lim_args, int side, double *F);
+/* plm cons edge states of one cell along one axis (plm.cl:27-91) */
+void oracle_plm_cons(int slope_limiter, int nint, int nstates,
+	const double *UL, const double *U, const double *UR, double *L, double *R);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,171 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+FP64 results must agree with the oracle to <= 1e-12 relative L1 per conserved
+variable; FP32-bulk results to <= 1e-5 against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l1
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def _pair(init, nsteps, **kw):
+    import pyoracle
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as init_euler
+    s = HipFVSolver(**kw)
+    Uint = init(s.cfg)
+    U0 = init_euler.to_ghosted(s.cfg, Uint)
+    s.resetState(U0.astype(s.dtype))
+    o = pyoracle.Oracle(pyoracle.make_cfg(**_oracle_kw(kw)))
+    o.set_U(U0)
+    o.boundary()
+    o.constrain_U()
+    return s, o
+
+
+def _oracle_kw(kw):
+    m = {'dim': 'dim', 'gridSize': 'size', 'mins': 'mins', 'maxs': 'maxs', 'usePLM': 'plm',
+         'slopeLimiter': 'slope_limiter', 'fluxLimiter': 'flux_limiter', 'integrator': 'integrator',
+         'cfl': 'cfl', 'boundary': 'boundary', 'fixedDT': 'fixed_dt', 'boundaryFixed': 'boundary_fixed'}
+    out = {m[k]: v for k, v in kw.items() if k in m}
+    out['precision'] = 'double'
+    return out
+
+
+def _interior(dim, U):
+    sl = [slice(None)] * 4
+    for i in range(dim):
+        sl[2 - i] = slice(2, -2)
+    return U[tuple(sl)]
+
+
+def _kh(seed=0):
+    from hydro.init import euler as init_euler
+    return lambda cfg: init_euler.kelvin_helmholtz(cfg, seed=seed)
+
+
+def _sod(cfg):
+    from hydro.init import euler as init_euler
+    return init_euler.sod(cfg)
+
+
+def _quad(seed):
+    from hydro.init import euler as init_euler
+    rng = np.random.default_rng(seed)
+    states = init_euler.random_quadrant_states(rng)
+    return lambda cfg: init_euler.quadrant(cfg, states)
+
+
+def _check(s, o, dim, tol, nsteps, label):
+    s.update(nsteps)
+    o.update(nsteps)
+    got = _interior(dim, s.UBufObj.toCPU())[..., :5]
+    ref = _interior(dim, o.U)[..., :5]
+    err = rel_l1(got, ref)
+    print(f'{label}: rel L1 per field {err}')
+    assert np.all(np.isfinite(got))
+    assert np.all(err <= tol), f'{label}: {err}'
+    assert abs(s.t - o.t) <= max(1e-12, tol) * abs(o.t)
+    return err
+
+
+def test_sod_1d_superbee_fe():
+    """1D Sod, Roe + superbee flux limiter, forward Euler, cfl .5, fixed ends, 100 steps"""
+    left = [1., 0, 0, 0, 1. / .4, 0]
+    right = [.125, 0, 0, 0, .1 / .4, 0]
+    kw = dict(dim=1, gridSize=256, fluxLimiter='superbee', integrator='forward Euler', cfl=.5,
+              boundary={'xmin': 'fixed', 'xmax': 'fixed'}, boundaryFixed={'xmin': left, 'xmax': right})
+    s, o = _pair(_sod, 100, **kw)
+    _check(s, o, 1, TOL64, 100, 'sod1d')
+
+
+@pytest.mark.parametrize('n', [(64, 64), (70, 37), (130, 9)])
+def test_kh_2d_plm_rk2_periodic(n):
+    kw = dict(dim=2, gridSize=n, mins=(-.5, -.5), maxs=(.5, .5), usePLM='plm cons', slopeLimiter='minmod',
+              integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic')
+    s, o = _pair(_kh(1), 20, **kw)
+    _check(s, o, 2, TOL64, 20, f'kh{n}')
+
+
+@pytest.mark.parametrize('bc', ['freeflow', 'mirror'])
+def test_quadrant_2d_plm_bcs(bc):
+    kw = dict(dim=2, gridSize=(61, 66), usePLM='plm cons', slopeLimiter='minmod',
+              integrator='Runge-Kutta 2, TVD', cfl=.4, boundary=bc)
+    s, o = _pair(_quad(3), 15, **kw)
+    _check(s, o, 2, TOL64, 15, f'quad-{bc}')
+
+
+@pytest.mark.parametrize('lim', ['superbee', 'van Leer', 'minmod', 'Koren'])
+def test_quadrant_2d_flux_limiter(lim):
+    kw = dict(dim=2, gridSize=(48, 40), fluxLimiter=lim, integrator='Runge-Kutta 3, TVD', cfl=.3,
+              boundary='freeflow')
+    s, o = _pair(_quad(4), 10, **kw)
+    _check(s, o, 2, TOL64, 10, f'fluxlim-{lim}')
+
+
+@pytest.mark.parametrize('integ', ['forward Euler', 'Runge-Kutta 4', 'Runge-Kutta 2 Heun',
+                                   'Runge-Kutta 4, TVD', 'Runge-Kutta 3'])
+def test_integrators(integ):
+    kw = dict(dim=2, gridSize=(40, 40), usePLM='plm cons', slopeLimiter='monotized central',
+              integrator=integ, cfl=.3, boundary='periodic', mins=(-.5, -.5), maxs=(.5, .5))
+    s, o = _pair(_kh(2), 6, **kw)
+    _check(s, o, 2, TOL64, 6, f'int-{integ}')
+
+
+def test_roe_no_limiter_1d():
+    kw = dict(dim=1, gridSize=200, integrator='Runge-Kutta 2, TVD', cfl=.5, boundary='mirror')
+    s, o = _pair(_sod, 40, **kw)
+    _check(s, o, 1, TOL64, 40, 'roe1d')
+
+
+def test_calc_dt_and_host_step():
+    kw = dict(dim=2, gridSize=(64, 32), usePLM='plm cons', slopeLimiter='minmod',
+              integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='freeflow')
+    s, o = _pair(_quad(5), 0, **kw)
+    dt_s = s.calcDT()
+    dt_o = o.calc_dt()
+    assert dt_s == dt_o
+    s.step(dt_s)
+    o.step(dt_o)
+    got = _interior(2, s.UBufObj.toCPU())[..., :5]
+    assert np.all(rel_l1(got, _interior(2, o.U)[..., :5]) <= TOL64)
+
+
+def test_download_ghosts_match_boundary():
+    """ghost cells of the download equal the oracle's after boundary()"""
+    for bc in ['periodic', 'mirror', 'freeflow']:
+        kw = dict(dim=2, gridSize=(13, 11), usePLM='plm cons', slopeLimiter='minmod',
+                  integrator='Runge-Kutta 2, TVD', cfl=.4, boundary=bc)
+        s, o = _pair(_quad(6), 0, **kw)
+        s.update(2)
+        o.update(2)
+        got = s.UBufObj.toCPU()[..., :5]
+        ref = o.U[..., :5]
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref)), bc
+
+
+def test_conserved_totals_periodic():
+    kw = dict(dim=2, gridSize=(64, 64), mins=(-.5, -.5), maxs=(.5, .5), usePLM='plm cons',
+              slopeLimiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic')
+    s, o = _pair(_kh(0), 0, **kw)
+    t0 = s.conservedTotals()
+    ref0 = _interior(2, o.U)[..., :5].reshape(-1, 5).sum(axis=0)
+    assert np.allclose(t0, ref0, rtol=1e-14, atol=1e-12)
+    s.update(10)
+    t1 = s.conservedTotals()
+    # periodic fluxes telescope: mass, momentum, energy are conserved to roundoff
+    assert np.allclose(t1[[0, 1, 4]], t0[[0, 1, 4]], rtol=1e-12, atol=1e-11)
+
+
+def test_kh_fp32_vs_fp64_oracle():
+    kw = dict(dim=2, gridSize=(128, 128), mins=(-.5, -.5), maxs=(.5, .5), usePLM='plm cons',
+              slopeLimiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic',
+              real='float')
+    s, o = _pair(_kh(0), 50, **kw)
+    _check(s, o, 2, TOL32, 50, 'kh-fp32')

@@ -5,6 +5,7 @@
 
 #include "fv_common.h"
 #include "fv_euler.h"
+#include "fv_glm.h"
 
 namespace hydro {
 
@@ -23,10 +24,16 @@ struct GridArgs {
 };
 
 static __global__ void reset_slots_kernel(DevScalars *dev, int mask) {
-	if (threadIdx.x == 0 && blockIdx.x == 0) {
+	if (blockIdx.x == 0 && threadIdx.x < DT_LANES) {
 		for (int i = 0; i < 3; ++i)
-			if (mask & (1 << i)) dev->dtmin[i] = INFINITY;
+			if (mask & (1 << i)) dev->dtmin[i][threadIdx.x] = INFINITY;
 	}
+}
+
+// min of the 64 sub-slots into sub-slot 0 (for host readback)
+static __global__ void fold_slot_kernel(DevScalars *dev, int slot) {
+	double m = read_dt_slot(dev, slot);
+	if (threadIdx.x == 0) dev->dtmin[slot][0] = m;
 }
 
 // calcDT (eqn.lua:1187-1224 + calcDT.cl:38-73) and reduceMin: per-cell dt in
@@ -42,11 +49,12 @@ __global__ void __launch_bounds__(256) dt_kernel(const GridArgs<R> g) {
 		for (int j = 0; j < Q; ++j) u[j] = (double)g.U[j * g.ps + c];
 #pragma unroll
 		for (int j = 0; j < E::NSTAT; ++j) u[Q + j] = (double)g.stat[j * g.ps + c];
-		double d = E::template dt_cell<double, DIM>(g.gamma, g.rhoMin, g.PMin, g.dx, u, g.divPhi, g.divPsi);
+		double d = E::template dt_cell<DIM>(g.gamma, g.rhoMin, g.PMin, g.dx, u, g.divPhi, g.divPsi);
 		m = fmin(m, d);
 	}
 	m = wave_min(m);
-	if ((threadIdx.x & 63) == 0 && m < INFINITY) atomic_min_pos(&g.dev->dtmin[g.slot], m);
+	int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+	if ((threadIdx.x & 63) == 0 && m < INFINITY) atomic_min_pos(&g.dev->dtmin[g.slot][w & (DT_LANES - 1)], m);
 }
 
 // constrainU (euler.cl:697-717) over the interior; ghosts follow by mapping.
@@ -58,7 +66,7 @@ __global__ void __launch_bounds__(256) constrain_kernel(const GridArgs<R> g) {
 		double u[Q];
 #pragma unroll
 		for (int j = 0; j < Q; ++j) u[j] = (double)g.U[j * g.ps + c];
-		E::template constrain<double>(g.gamma, g.rhoMin, g.PMin, u);
+		E::constrain(g.gamma, g.rhoMin, g.PMin, u);
 #pragma unroll
 		for (int j = 0; j < Q; ++j) g.U[j * g.ps + c] = R(u[j]);
 	}

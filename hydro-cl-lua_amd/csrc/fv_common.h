@@ -16,11 +16,12 @@ constexpr int STRIP_OUT = WAVE - 4;// output columns per wave (2-cell halo each 
 
 // Device-resident per-solver scalars: dt minima rotate through three slots so
 // a step can read one, reduce into the next and clear the third.
+constexpr int DT_LANES = 64;       // dt minima are spread over 64 addresses
 struct DevScalars {
-	double dtmin[3];
+	double dtmin[3][DT_LANES];
 	double t;
 	double dt;
-	double pad[3];
+	double pad[6];
 };
 
 // Boundary description of one axis, resolved on the host.
@@ -52,6 +53,7 @@ struct StageArgs {
 	int derivOn;              // volume > 1e-7 (fvsolver.cl:97)
 	int fluxOn[3];            // cell_area > 1e-7 (fvsolver.lua:105)
 	R dx[3];
+	R invdx[3];               // 1 / dx in the stored precision (glm-maxwell.cl:351)
 	double dxD[3];
 	// dt: device slot (cfl * dtmin[slot]) or a host value
 	DevScalars *dev;
@@ -141,6 +143,12 @@ __device__ __forceinline__ double wave_min(double v) {
 		v = o < v ? o : v;
 	}
 	return v;
+}
+
+// min over the 64 sub-slots of a dt slot: one coalesced load per wave
+__device__ __forceinline__ double read_dt_slot(const DevScalars *d, int slot) {
+	double v = d->dtmin[slot][threadIdx.x & 63];
+	return wave_min(v);
 }
 
 __device__ __forceinline__ void atomic_min_pos(double *p, double v) {

@@ -3,218 +3,264 @@
 // Arithmetic follows hydro/eqn/euler.cl with Cartesian normals resolved at
 // compile time: the reference multiplies by normal components that are the
 // constants 0 and 1 (coord/coord.lua:2376-2385); here those terms are dropped,
-// which changes nothing but the sign of an exact zero. Evaluation order, every
-// division and every square root are kept as the reference writes them.
+// which changes nothing but the sign of an exact zero. Evaluation order is kept
+// as the reference writes it; divisions and square roots go through the Math
+// policy (IEEE, or the faithful FAST sequences).
+//
+// NVEL < 3 means velocity components NVEL..2 are identically zero (2D / 1D
+// runs whose initial state has no out-of-plane momentum). Every term such a
+// component enters is an exact zero in the reference too (its wave amplitude
+// L dU and its flux are +-0), so those terms are dropped and its plane is not
+// streamed.
 #pragma once
 
 #include "fv_common.h"
+#include "fv_math.h"
 
 namespace hydro {
 
-template <typename R>
-struct Euler {
-	static constexpr int Q = 5;       // numIntStates (euler.lua:14)
-	static constexpr int NSTAT = 0;   // no static fields on the device
-	static constexpr int NW = 5;      // numWaves (euler.lua:13)
-	static constexpr int NSTATES = 6; // cons_t reals incl. ePot
+template <bool FAST, int NVEL>
+struct EulerT {
+	template <typename R>
+	struct Eqn {
+		using M = Math<R, FAST>;
+		static constexpr int Q = 5;       // numIntStates (euler.lua:14)
+		static constexpr int NSTAT = 0;   // no static fields on the device
+		static constexpr int NW = 5;      // numWaves (euler.lua:13)
+		static constexpr int NSTATES = 6; // cons_t reals incl. ePot
+		static constexpr bool SOURCE = false;
 
-	struct Par {
-		R gamma, g1, rhoMin, PMin;
-	};
-	struct Prim {
-		R rho, v[3], P;
-	};
-	struct Eig {
-		R rho, v[3], hTotal, Cs, vSq;   // vL == v for Cartesian
-	};
+		// fields the kernel streams: rho, the active momenta, ETotal
+		__device__ static constexpr bool active(int f) { return f == 0 || f == 4 || (f >= 1 && f <= 3 && f - 1 < NVEL); }
 
-	__device__ static __forceinline__ Par par(const StageArgs<R> &a) {
-		Par p;
-		p.gamma = a.gamma;
-		p.g1 = a.gamma - R(1);
-		p.rhoMin = a.rhoMin;
-		p.PMin = a.PMin;
-		return p;
-	}
+		struct Par {
+			R gamma, g1, rhoMin, PMin;
+		};
+		struct Prim {
+			R rho, v[3], P, inv;   // inv = 1 / rho
+		};
+		struct Eig {
+			R rho, v[3], hTotal, Cs, vSq;   // vL == v for Cartesian
+		};
 
-	// primFromCons (euler.cl:140-159; the rhoMin branch is dead code there)
-	__device__ static __forceinline__ Prim prim(const Par &p, const R *U) {
-		Prim W;
-		R inv = R(1) / U[0];
-		W.rho = U[0];
-		W.v[0] = U[1] * inv;
-		W.v[1] = U[2] * inv;
-		W.v[2] = U[3] * inv;
-		R lenSq = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
-		R EKin = U[0] < p.rhoMin ? R(0) : (R(.5) * lenSq / U[0]);
-		W.P = U[0] < p.rhoMin ? R(0) : (p.g1 * (U[4] - EKin));
-		return W;
-	}
-
-	__device__ static __forceinline__ R calc_Cs(const Par &p, const Prim &W) {
-		if (W.P <= p.PMin) return R(0);
-		if (W.rho < p.rhoMin) return R(INFINITY);
-		return sqrt(p.gamma * W.P / W.rho);
-	}
-
-	// eigen_forInterface (euler.cl:347-430) from the two states and their prims
-	__device__ static __forceinline__ Eig eigen(const Par &p, const R *UL, const R *UR,
-		const Prim &WL, const Prim &WR)
-	{
-		const R rhoEpsilon = R(1e-5);
-		Eig e;
-		if (UL[0] < rhoEpsilon && UR[0] < rhoEpsilon) {
-			e.rho = 0; e.v[0] = e.v[1] = e.v[2] = 0; e.hTotal = 0; e.Cs = 0; e.vSq = 0;
-			return e;
+		__device__ static __forceinline__ Par par(const StageArgs<R> &a) {
+			Par p;
+			p.gamma = a.gamma;
+			p.g1 = a.gamma - R(1);
+			p.rhoMin = a.rhoMin;
+			p.PMin = a.PMin;
+			return p;
 		}
-		if (UL[0] < rhoEpsilon || UR[0] < rhoEpsilon) {
-			bool useR = UL[0] < rhoEpsilon;
-			const R *Us = useR ? UR : UL;
-			const Prim &W = useR ? WR : WL;
-			e.rho = Us[0];
-			e.v[0] = W.v[0]; e.v[1] = W.v[1]; e.v[2] = W.v[2];
-			e.vSq = W.v[0] * W.v[0] + W.v[1] * W.v[1] + W.v[2] * W.v[2];
-			e.hTotal = (W.P + Us[4]) / W.rho;
-			e.Cs = calc_Cs(p, W);
-			return e;
+
+		template <typename T>
+		__device__ static __forceinline__ T lenSq3(const T *m) {
+			T s = m[0] * m[0];
+			if constexpr (NVEL > 1) s = s + m[1] * m[1];
+			if constexpr (NVEL > 2) s = s + m[2] * m[2];
+			return s;
 		}
-		R sqrtRhoL = sqrt(WL.rho);
-		R hTotalL = (WL.P + UL[4]) / WL.rho;
-		R sqrtRhoR = sqrt(WR.rho);
-		R hTotalR = (WR.P + UR[4]) / WR.rho;
-		R invDenom = R(1) / (sqrtRhoL + sqrtRhoR);
-		e.rho = sqrtRhoL * sqrtRhoR;
-		R wl = sqrtRhoL * invDenom, wr = sqrtRhoR * invDenom;
-		for (int i = 0; i < 3; ++i) e.v[i] = WL.v[i] * wl + WR.v[i] * wr;
-		R hTotal = invDenom * (sqrtRhoL * hTotalL + sqrtRhoR * hTotalR);
-		R vSq = e.v[0] * e.v[0] + e.v[1] * e.v[1] + e.v[2] * e.v[2];
-		R eKin = R(.5) * vSq;
-		R h = hTotal - eKin;
-		if (h < rhoEpsilon) {
-			e.hTotal = eKin;
-			e.Cs = 0;
-		} else {
-			e.hTotal = hTotal;
-			e.Cs = sqrt(p.g1 * h);
+
+		// primFromCons (euler.cl:140-159; the rhoMin branch is dead code there)
+		__device__ static __forceinline__ Prim prim(const Par &p, const R *U) {
+			Prim W;
+			R inv = M::rcp(U[0]);
+			W.inv = inv;
+			W.rho = U[0];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) W.v[i] = i < NVEL ? U[1 + i] * inv : R(0);
+			R lenSq = lenSq3(U + 1);
+			R EKin = U[0] < p.rhoMin ? R(0) : M::div_inv(R(.5) * lenSq, U[0], inv);
+			W.P = U[0] < p.rhoMin ? R(0) : (p.g1 * (U[4] - EKin));
+			return W;
 		}
-		e.vSq = vSq;
-		return e;
-	}
 
-	// eigen_leftTransform (euler.cl:434-489), normal along axis S
-	template <int S>
-	__device__ static __forceinline__ void left(const Par &p, const Eig &e, const R *X, R *Y) {
-		if (e.rho < p.rhoMin) {
-			for (int i = 0; i < 5; ++i) Y[i] = X[i];
-			return;
+		__device__ static __forceinline__ R calc_Cs(const Par &p, const Prim &W) {
+			if (W.P <= p.PMin) return R(0);
+			if (W.rho < p.rhoMin) return R(INFINITY);
+			return M::sqrt_(M::div_inv(p.gamma * W.P, W.rho, W.inv));
 		}
-		constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
-		const R Cs = e.Cs, vSq = e.vSq, g1 = p.g1;
-		R denom = R(2) * Cs * Cs;
-		R invDenom = R(1) / denom;
-		R hg1vSq = R(.5) * g1 * vSq;
-		R Csvn = Cs * e.v[A];
-		R t[3];
-		for (int i = 0; i < 3; ++i) t[i] = -g1 * e.v[i];
-		R m[3] = {t[0], t[1], t[2]};
-		m[A] = t[A] - Cs;   // -g1 vL_a - Cs n1_a
-		R pl[3] = {t[0], t[1], t[2]};
-		pl[A] = t[A] + Cs;  // -g1 vL_a + Cs n1_a
-		Y[0] = (X[0] * (hg1vSq + Csvn) + X[1] * m[0] + X[2] * m[1] + X[3] * m[2] + X[4] * g1) * invDenom;
-		Y[1] = (X[0] * (denom - g1 * vSq)
-			+ X[1] * R(2) * g1 * e.v[0]
-			+ X[2] * R(2) * g1 * e.v[1]
-			+ X[3] * R(2) * g1 * e.v[2]
-			+ X[4] * R(-2) * g1) * invDenom;
-		Y[2] = X[0] * -e.v[B] + X[1 + B];
-		Y[3] = X[0] * -e.v[C] + X[1 + C];
-		Y[4] = (X[0] * (hg1vSq - Csvn) + X[1] * pl[0] + X[2] * pl[1] + X[3] * pl[2] + X[4] * g1) * invDenom;
-	}
 
-	// eigen_rightTransform (euler.cl:493-542)
-	template <int S>
-	__device__ static __forceinline__ void right(const Par &p, const Eig &e, const R *X, R *Y) {
-		if (e.rho < p.rhoMin) {
-			for (int i = 0; i < 5; ++i) Y[i] = X[i];
-			return;
-		}
-		constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
-		const R Cs = e.Cs;
-		Y[0] = X[0] + X[1] + X[4];
-		Y[1 + A] = X[0] * (e.v[A] - Cs) + X[1] * e.v[A] + X[4] * (e.v[A] + Cs);
-		Y[1 + B] = X[0] * e.v[B] + X[1] * e.v[B] + X[2] + X[4] * e.v[B];
-		Y[1 + C] = X[0] * e.v[C] + X[1] * e.v[C] + X[3] + X[4] * e.v[C];
-		R Csvn = Cs * e.v[A];
-		Y[4] = X[0] * (e.hTotal - Csvn) + X[1] * R(.5) * e.vSq + X[2] * e.v[B] + X[3] * e.v[C]
-			+ X[4] * (e.hTotal + Csvn);
-	}
-
-	// fluxFromCons (euler.cl:273-291) given the state's prim
-	template <int S>
-	__device__ static __forceinline__ void flux(const R *U, const Prim &W, R *F) {
-		R vn = W.v[S];
-		F[0] = U[0] * vn;
-		F[1] = U[1] * vn;
-		F[2] = U[2] * vn;
-		F[3] = U[3] * vn;
-		F[1 + S] = F[1 + S] + W.P;
-		F[4] = (U[4] + W.P) * vn;
-	}
-
-	// eigenWaveCode (euler.lua:309-325)
-	template <int S>
-	__device__ static __forceinline__ void waves(const Eig &e, R *l) {
-		R vn = e.v[S];
-		l[0] = vn - e.Cs;
-		l[1] = vn;
-		l[2] = vn;
-		l[3] = vn;
-		l[4] = vn + e.Cs;
-	}
-
-	// constrainU (euler.cl:697-717) on a state in type T
-	template <typename T>
-	__device__ static __forceinline__ void constrain(T gamma, T rhoMin, T PMin, T *U) {
-		if (U[0] < rhoMin) U[0] = rhoMin;
-		T g1 = gamma - T(1);
-		T inv = T(1) / U[0];
-		T v0 = U[1] * inv, v1 = U[2] * inv, v2 = U[3] * inv;
-		T lenSq = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
-		T EKin = U[0] < rhoMin ? T(0) : (T(.5) * lenSq / U[0]);
-		T P = U[0] < rhoMin ? T(0) : (g1 * (U[4] - EKin));
-		if (P < PMin) P = PMin;
-		T rho = U[0];
-		U[1] = v0 * rho;
-		U[2] = v1 * rho;
-		U[3] = v2 * rho;
-		U[4] = rho * (T(.5) * (v0 * v0 + v1 * v1 + v2 * v2)) + P / g1;
-	}
-
-	// calcDTCell (calcDT.cl:38-73, euler.lua:346-373) on a state in type T
-	template <typename T, int DIM>
-	__device__ static __forceinline__ T dt_cell(T gamma, T rhoMin, T PMin, const T *dx, const T *U, T, T) {
-		T lenSq = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
-		T EKin = U[0] < rhoMin ? T(0) : (T(.5) * lenSq / U[0]);
-		T P = U[0] < rhoMin ? T(0) : ((gamma - T(1)) * (U[4] - EKin));
-		T Cs;
-		if (P <= PMin) Cs = 0;
-		else if (U[0] < rhoMin) Cs = T(INFINITY);
-		else Cs = sqrt(gamma * P / U[0]);
-		T dt = T(INFINITY);
-		for (int s = 0; s < DIM; ++s) {
-			if (dx[s] > T(0)) {
-				T vn = U[0] < rhoMin ? T(0) : U[1 + s] / U[0];
-				T lmin = vn - Cs, lmax = vn + Cs;
-				T absLambdaMax = fmax(fabs(lmin), fabs(lmax));
-				absLambdaMax = fmax(T(1e-9), absLambdaMax);
-				dt = fmin(dt, dx[s] / absLambdaMax);
+		// eigen_forInterface (euler.cl:347-430) from the two states and their prims
+		__device__ static __forceinline__ Eig eigen(const Par &p, const R *UL, const R *UR,
+			const Prim &WL, const Prim &WR)
+		{
+			const R rhoEpsilon = R(1e-5);
+			Eig e;
+			if (UL[0] < rhoEpsilon && UR[0] < rhoEpsilon) {
+				e.rho = 0; e.v[0] = e.v[1] = e.v[2] = 0; e.hTotal = 0; e.Cs = 0; e.vSq = 0;
+				return e;
 			}
+			if (UL[0] < rhoEpsilon || UR[0] < rhoEpsilon) {
+				bool useR = UL[0] < rhoEpsilon;
+				const R *Us = useR ? UR : UL;
+				const Prim &W = useR ? WR : WL;
+				e.rho = Us[0];
+				e.v[0] = W.v[0]; e.v[1] = W.v[1]; e.v[2] = W.v[2];
+				e.vSq = lenSq3(W.v);
+				e.hTotal = M::div_inv(W.P + Us[4], W.rho, W.inv);
+				e.Cs = calc_Cs(p, W);
+				return e;
+			}
+			R sqrtRhoL = M::sqrt_(WL.rho);
+			R hTotalL = M::div_inv(WL.P + UL[4], WL.rho, WL.inv);
+			R sqrtRhoR = M::sqrt_(WR.rho);
+			R hTotalR = M::div_inv(WR.P + UR[4], WR.rho, WR.inv);
+			R invDenom = M::rcp(sqrtRhoL + sqrtRhoR);
+			e.rho = sqrtRhoL * sqrtRhoR;
+			R wl = sqrtRhoL * invDenom, wr = sqrtRhoR * invDenom;
+#pragma unroll
+			for (int i = 0; i < 3; ++i) e.v[i] = i < NVEL ? WL.v[i] * wl + WR.v[i] * wr : R(0);
+			R hTotal = invDenom * (sqrtRhoL * hTotalL + sqrtRhoR * hTotalR);
+			R vSq = lenSq3(e.v);
+			R eKin = R(.5) * vSq;
+			R h = hTotal - eKin;
+			if (h < rhoEpsilon) {
+				e.hTotal = eKin;
+				e.Cs = 0;
+			} else {
+				e.hTotal = hTotal;
+				e.Cs = M::sqrt_(p.g1 * h);
+			}
+			e.vSq = vSq;
+			return e;
 		}
-		return dt;
-	}
 
-	// mirror reflects the normal momentum component
-	__device__ static __forceinline__ bool reflects(int field, int axis) { return field == 1 + axis; }
+		// eigen_leftTransform (euler.cl:434-489), normal along axis S
+		template <int S>
+		__device__ static __forceinline__ void left(const Par &p, const Eig &e, const R *X, R *Y) {
+			if (e.rho < p.rhoMin) {
+				for (int i = 0; i < 5; ++i) Y[i] = X[i];
+				return;
+			}
+			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
+			const R Cs = e.Cs, vSq = e.vSq, g1 = p.g1;
+			R denom = R(2) * Cs * Cs;
+			R invDenom = M::rcp(denom);
+			R hg1vSq = R(.5) * g1 * vSq;
+			R Csvn = Cs * e.v[A];
+			R t[3];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) t[i] = -g1 * e.v[i];
+			// (X0 (.5 g1 vSq +- Cs vn) + sum_i X_{1+i} (-g1 vL_i -+ Cs n1_i) + X4 g1) / (2 Cs^2)
+			R y0 = X[0] * (hg1vSq + Csvn);
+			R y4 = X[0] * (hg1vSq - Csvn);
+#pragma unroll
+			for (int i = 0; i < 3; ++i) {
+				if (i < NVEL || i == A) {
+					R cm = i == A ? t[i] - Cs : t[i];
+					R cp = i == A ? t[i] + Cs : t[i];
+					y0 = y0 + X[1 + i] * cm;
+					y4 = y4 + X[1 + i] * cp;
+				}
+			}
+			Y[0] = (y0 + X[4] * g1) * invDenom;
+			Y[4] = (y4 + X[4] * g1) * invDenom;
+			R y1 = X[0] * (denom - g1 * vSq);
+#pragma unroll
+			for (int i = 0; i < 3; ++i)
+				if (i < NVEL) y1 = y1 + X[1 + i] * R(2) * g1 * e.v[i];
+			Y[1] = (y1 + X[4] * R(-2) * g1) * invDenom;
+			Y[2] = B < NVEL ? X[0] * -e.v[B] + X[1 + B] : R(0);
+			Y[3] = C < NVEL ? X[0] * -e.v[C] + X[1 + C] : R(0);
+		}
+
+		// eigen_rightTransform (euler.cl:493-542)
+		template <int S>
+		__device__ static __forceinline__ void right(const Par &p, const Eig &e, const R *X, R *Y) {
+			if (e.rho < p.rhoMin) {
+				for (int i = 0; i < 5; ++i) Y[i] = X[i];
+				return;
+			}
+			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
+			const R Cs = e.Cs;
+			Y[0] = X[0] + X[1] + X[4];
+			Y[1 + A] = X[0] * (e.v[A] - Cs) + X[1] * e.v[A] + X[4] * (e.v[A] + Cs);
+			if constexpr (B < NVEL) Y[1 + B] = X[0] * e.v[B] + X[1] * e.v[B] + X[2] + X[4] * e.v[B];
+			else Y[1 + B] = R(0);
+			if constexpr (C < NVEL) Y[1 + C] = X[0] * e.v[C] + X[1] * e.v[C] + X[3] + X[4] * e.v[C];
+			else Y[1 + C] = R(0);
+			R Csvn = Cs * e.v[A];
+			R y4 = X[0] * (e.hTotal - Csvn) + X[1] * R(.5) * e.vSq;
+			if constexpr (B < NVEL) y4 = y4 + X[2] * e.v[B];
+			if constexpr (C < NVEL) y4 = y4 + X[3] * e.v[C];
+			Y[4] = y4 + X[4] * (e.hTotal + Csvn);
+		}
+
+		// fluxFromCons (euler.cl:273-291) given the state's prim
+		template <int S>
+		__device__ static __forceinline__ void flux(const Par &, const R *U, const Prim &W, R *F) {
+			R vn = W.v[S];
+			F[0] = U[0] * vn;
+#pragma unroll
+			for (int i = 0; i < 3; ++i) F[1 + i] = i < NVEL ? U[1 + i] * vn : R(0);
+			F[1 + S] = F[1 + S] + W.P;
+			F[4] = (U[4] + W.P) * vn;
+		}
+
+		// eigenWaveCode (euler.lua:309-325)
+		template <int S>
+		__device__ static __forceinline__ void waves(const Par &, const Eig &e, R *l) {
+			R vn = e.v[S];
+			l[0] = vn - e.Cs;
+			l[1] = vn;
+			l[2] = vn;
+			l[3] = vn;
+			l[4] = vn + e.Cs;
+		}
+
+		// constrainU (euler.cl:697-717) on a state in double
+		__device__ static __forceinline__ void constrain(double gamma, double rhoMin, double PMin, double *U) {
+			using MD = Math<double, FAST>;
+			if (U[0] < rhoMin) U[0] = rhoMin;
+			double g1 = gamma - 1.;
+			double inv = MD::rcp(U[0]);
+			double v[3];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) v[i] = i < NVEL ? U[1 + i] * inv : 0.;
+			double lenSq = lenSq3(U + 1);
+			double EKin = U[0] < rhoMin ? 0. : MD::div_inv(.5 * lenSq, U[0], inv);
+			double P = U[0] < rhoMin ? 0. : (g1 * (U[4] - EKin));
+			if (P < PMin) P = PMin;
+			double rho = U[0];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) U[1 + i] = i < NVEL ? v[i] * rho : 0.;
+			U[4] = rho * (.5 * lenSq3(v)) + MD::div(P, g1);
+		}
+
+		// calcDTCell (calcDT.cl:38-73, euler.lua:346-373) in double
+		template <int DIM>
+		__device__ static __forceinline__ double dt_cell(double gamma, double rhoMin, double PMin,
+			const double *dx, const double *U, double, double)
+		{
+			using MD = Math<double, FAST>;
+			double inv = MD::rcp(U[0]);
+			double lenSq = lenSq3(U + 1);
+			double EKin = U[0] < rhoMin ? 0. : MD::div_inv(.5 * lenSq, U[0], inv);
+			double P = U[0] < rhoMin ? 0. : ((gamma - 1.) * (U[4] - EKin));
+			double Cs;
+			if (P <= PMin) Cs = 0;
+			else if (U[0] < rhoMin) Cs = INFINITY;
+			else Cs = MD::sqrt_(MD::div_inv(gamma * P, U[0], inv));
+			double dt = INFINITY;
+#pragma unroll
+			for (int s = 0; s < DIM; ++s) {
+				if (dx[s] > 0.) {
+					double vn = U[0] < rhoMin ? 0. : (s < NVEL ? MD::div_inv(U[1 + s], U[0], inv) : 0.);
+					double lmin = vn - Cs, lmax = vn + Cs;
+					double absLambdaMax = fmax(fabs(lmin), fabs(lmax));
+					absLambdaMax = fmax(1e-9, absLambdaMax);
+					dt = fmin(dt, MD::div(dx[s], absLambdaMax));
+				}
+			}
+			return dt;
+		}
+
+		template <int DIM>
+		__device__ static __forceinline__ void source(const Par &, const R *, const R *, const R *, const R *, R *) {}
+
+		// mirror reflects the normal momentum component
+		__device__ static __forceinline__ bool reflects(int field, int axis) { return field == 1 + axis; }
+	};
 };
 
 } // namespace hydro

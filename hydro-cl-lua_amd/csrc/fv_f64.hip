@@ -8,4 +8,8 @@ hipError_t launch_reset_slots(DevScalars *dev, int mask, hipStream_t s) {
 	hipLaunchKernelGGL(reset_slots_kernel, dim3(1), dim3(64), 0, s, dev, mask);
 	return hipGetLastError();
 }
+hipError_t launch_fold_slot(DevScalars *dev, int slot, hipStream_t s) {
+	hipLaunchKernelGGL(fold_slot_kernel, dim3(1), dim3(64), 0, s, dev, slot);
+	return hipGetLastError();
+}
 } // namespace hydro

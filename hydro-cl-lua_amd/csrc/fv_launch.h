@@ -17,6 +17,8 @@ struct StageKey {
 	int plm;        // plm cons on
 	int lim;        // Roe flux limiter on
 	int limiter;    // limiter index (flux limiter if lim, slope limiter if plm)
+	int fast;       // FAST math policy
+	int nvel;       // Euler: active velocity components (dim or 3)
 };
 
 // Limiters with a compile-time form; the rest use the runtime switch.
@@ -25,7 +27,6 @@ inline int limiter_template_id(int index) {
 	case HYDRO_LIMITER_MINMOD:
 	case HYDRO_LIMITER_SUPERBEE:
 	case HYDRO_LIMITER_VAN_LEER:
-	case HYDRO_LIMITER_MONOTIZED_CENTRAL:
 		return index;
 	}
 	return -1;
@@ -37,14 +38,15 @@ template <typename R>
 hipError_t launch_stage(const StageKey &k, const StageArgs<R> &a, dim3 grid, dim3 block, hipStream_t s);
 
 template <typename R>
-hipError_t launch_dt(int eqn, int dim, const GridArgs<R> &g, hipStream_t s);
+hipError_t launch_dt(int eqn, int dim, int fast, const GridArgs<R> &g, hipStream_t s);
 
 template <typename R>
-hipError_t launch_constrain(int eqn, const GridArgs<R> &g, hipStream_t s);
+hipError_t launch_constrain(int eqn, int fast, const GridArgs<R> &g, hipStream_t s);
 
 template <typename R>
 hipError_t launch_totals(const R *U, long ps, long ncell, int q, KSum *partial, int nPartial, double *out, hipStream_t s);
 
 hipError_t launch_reset_slots(DevScalars *dev, int mask, hipStream_t s);
+hipError_t launch_fold_slot(DevScalars *dev, int slot, hipStream_t s);
 
 } // namespace hydro

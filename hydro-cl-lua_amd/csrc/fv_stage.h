@@ -17,6 +17,7 @@
 
 #include "fv_common.h"
 #include "fv_euler.h"
+#include "fv_glm.h"
 #include "fv_limiters.h"
 
 namespace hydro {
@@ -35,6 +36,10 @@ __device__ __forceinline__ void load_cell(const StageArgs<R> &a, const R *base, 
 	long off = (long)my.idx * a.step[1] + mx.idx;
 #pragma unroll
 	for (int f = 0; f < Cell<R, E>::NV; ++f) {
+		if (!E::active(f)) {
+			c.v[f] = R(0);
+			continue;
+		}
 		R v;
 		if (f < E::Q) v = base[f * a.ps + off];
 		else v = stat[(f - E::Q) * a.ps + off];
@@ -102,14 +107,14 @@ __device__ __forceinline__ void iface_prep(const typename E::Par &p, const Cell<
 	auto WL = E::prim(p, UL.v);
 	auto WR = E::prim(p, UR.v);
 	I.e = E::eigen(p, UL.v, UR.v, WL, WR);
-	E::template waves<S>(I.e, I.lambda);
+	E::template waves<S>(p, I.e, I.lambda);
 	R dU[E::Q];
 #pragma unroll
 	for (int j = 0; j < E::Q; ++j) dU[j] = UR.v[j] - UL.v[j];
 	E::template left<S>(p, I.e, dU, I.dW);
 	R FL[E::Q], FR[E::Q];
-	E::template flux<S>(UL.v, WL, FL);
-	E::template flux<S>(UR.v, WR, FR);
+	E::template flux<S>(p, UL.v, WL, FL);
+	E::template flux<S>(p, UR.v, WR, FR);
 #pragma unroll
 	for (int j = 0; j < E::Q; ++j) I.Favg[j] = R(.5) * (FL[j] + FR[j]);
 }
@@ -156,8 +161,8 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 	const int wave = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
 
 	// dt of this step (solverbase.lua:3016): cfl * min, or a host value
-	double dt = a.dtSlot >= 0 ? a.cfl * a.dev->dtmin[a.dtSlot] : a.hostDt;
-	if (blockIdx.x == 0 && threadIdx.x == 0 && a.resetSlot >= 0) a.dev->dtmin[a.resetSlot] = INFINITY;
+	double dt = a.dtSlot >= 0 ? a.cfl * read_dt_slot(a.dev, a.dtSlot) : a.hostDt;
+	if (blockIdx.x == 0 && threadIdx.x < DT_LANES && a.resetSlot >= 0) a.dev->dtmin[a.resetSlot][threadIdx.x] = INFINITY;
 
 	const int nx = a.n[0];
 	const int ny = DIM >= 2 ? a.n[1] : 1;
@@ -178,12 +183,12 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 		const int y1 = min(y0 + a.rowsPerWave, ny);
 
 		// y pipeline state
-		C W0, W1, W2;
+		C Wm, W0, W1, W2;       // rows r-1 .. r+2
 		C Ry0;                  // PLM: R edge of row r (from the previous iteration)
 		Iface<R, E> I0, I1;     // LIM: interfaces r-1/2 and r+1/2
 		R FaYprev[Q];
 #pragma unroll
-		for (int f = 0; f < C::NV; ++f) { W0.v[f] = W1.v[f] = W2.v[f] = Ry0.v[f] = R(0); }
+		for (int f = 0; f < C::NV; ++f) { Wm.v[f] = W0.v[f] = W1.v[f] = W2.v[f] = Ry0.v[f] = R(0); }
 #pragma unroll
 		for (int j = 0; j < Q; ++j) FaYprev[j] = R(0);
 		I0 = Iface<R, E>{};
@@ -280,6 +285,14 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 					if constexpr (DIM >= 2) deriv[j] = deriv[j] - dyterm[j];
 					if (!a.derivOn) deriv[j] = R(0);
 				}
+				if constexpr (E::SOURCE) {
+					// eqn addSource after calcDerivFromFlux (solverbase.lua:3208-3214)
+					R em[3], ep[3];
+					em[0] = from_left(U.v[10]);
+					ep[0] = from_right(U.v[10]);
+					if constexpr (DIM >= 2) { em[1] = Wm.v[10]; ep[1] = W1.v[10]; }
+					E::template source<DIM>(par, a.invdx, U.v, em, ep, deriv);
+				}
 
 				// ---- integrator combine (rk.lua:155-174, fe.lua:50), constrainU
 				if (outLane) {
@@ -293,6 +306,7 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 						bool fromSrc = Uk == a.src;
 #pragma unroll
 						for (int j = 0; j < Q; ++j) {
+							if (!E::active(j)) continue;
 							R u = fromSrc ? U.v[j] : Uk[j * a.ps + off];
 							acc[j] = acc[j] + (double)u * al;
 						}
@@ -301,23 +315,26 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 						const R *Dk = a.accD[k];
 						double c = (double)R(a.beta[k] * dt);
 #pragma unroll
-						for (int j = 0; j < Q; ++j) acc[j] = acc[j] + (double)Dk[j * a.ps + off] * c;
+						for (int j = 0; j < Q; ++j)
+							if (E::active(j)) acc[j] = acc[j] + (double)Dk[j * a.ps + off] * c;
 					}
 					if (a.storeD) {
 #pragma unroll
-						for (int j = 0; j < Q; ++j) a.storeD[j * a.ps + off] = deriv[j];
+						for (int j = 0; j < Q; ++j)
+							if (E::active(j)) a.storeD[j * a.ps + off] = deriv[j];
 					}
 					if (a.betaNew != 0.) {
 						double c = (double)R(a.betaNew * dt);
 #pragma unroll
-						for (int j = 0; j < Q; ++j) acc[j] = acc[j] + (double)deriv[j] * c;
+						for (int j = 0; j < Q; ++j)
+							if (E::active(j)) acc[j] = acc[j] + (double)deriv[j] * c;
 					}
-					E::template constrain<double>(a.gammaD, a.rhoMinD, a.PMinD, acc);
+					E::constrain(a.gammaD, a.rhoMinD, a.PMinD, acc);
 					R o[Q];
 #pragma unroll
 					for (int j = 0; j < Q; ++j) {
 						o[j] = R(acc[j]);
-						a.out[j * a.ps + off] = o[j];
+						if (E::active(j)) a.out[j * a.ps + off] = o[j];
 					}
 					if (a.reduceSlot >= 0) {
 						double ud[C::NV];
@@ -326,7 +343,7 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 						for (int j = 0; j < Q; ++j) ud[j] = (double)o[j];
 #pragma unroll
 						for (int j = Q; j < C::NV; ++j) ud[j] = (double)U.v[j];
-						double d = E::template dt_cell<double, DIM>(a.gammaD, a.rhoMinD,
+						double d = E::template dt_cell<DIM>(a.gammaD, a.rhoMinD,
 							a.PMinD, dxd, ud, a.divPhiD, a.divPsiD);
 						dtLocal = fmin(dtLocal, d);
 					}
@@ -335,6 +352,7 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 
 			// ---- advance the row window
 			if constexpr (DIM >= 2) {
+				if constexpr (E::SOURCE) Wm = W0;
 				if constexpr (PLM || LIM) { W0 = W1; W1 = W2; }
 				else { W0 = W1; }
 			}
@@ -343,7 +361,7 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 
 	if (a.reduceSlot >= 0) {
 		double m = wave_min(dtLocal);
-		if (lane == 0 && m < INFINITY) atomic_min_pos(&a.dev->dtmin[a.reduceSlot], m);
+		if (lane == 0 && m < INFINITY) atomic_min_pos(&a.dev->dtmin[a.reduceSlot][wave & (DT_LANES - 1)], m);
 	}
 	if (a.advanceTime && blockIdx.x == 0 && threadIdx.x == 0) {
 		a.dev->t = a.dev->t + dt;

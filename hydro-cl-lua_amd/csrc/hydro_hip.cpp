@@ -151,6 +151,7 @@ struct hydro_solver {
 	int dtValid;        // slot holds the min for the current state
 	int rowsPerWave;
 	int xcdRemap;
+	int nvel;           // Euler: velocity components that are not identically zero
 	std::vector<double> ePot;   // Euler ePot of the last upload (dynamics-neutral)
 	size_t devBytes;
 };
@@ -162,7 +163,7 @@ static int validate(const hydro_cfg *c) {
 	if (c->dim < 1 || c->dim > 2) return fail("dim %d not supported (1 or 2)", c->dim);
 	for (int i = 0; i < c->dim; ++i)
 		if (c->size[i] < 1) return fail("size[%d] = %d must be >= 1", i, c->size[i]);
-	if (c->eqn != HYDRO_EQN_EULER) return fail("eqn %d not supported by this build", c->eqn);
+	if (c->eqn != HYDRO_EQN_EULER && c->eqn != HYDRO_EQN_GLM_MAXWELL) return fail("eqn %d not supported", c->eqn);
 	if (c->flux != HYDRO_FLUX_ROE) return fail("flux %d not supported", c->flux);
 	if (c->flux_limiter < 0 || c->flux_limiter >= HYDRO_NUM_LIMITERS) return fail("flux_limiter %d out of range", c->flux_limiter);
 	if (c->slope_limiter < 0 || c->slope_limiter >= HYDRO_NUM_LIMITERS) return fail("slope_limiter %d out of range", c->slope_limiter);
@@ -212,6 +213,7 @@ extern "C" int hydro_create(const hydro_cfg *cfg, hydro_solver **out) {
 	h->elem = elem_size(*cfg);
 	rk_table(cfg->integrator, h->rk);
 	h->xcdRemap = 1;
+	h->nvel = 3;
 	choose_launch(h);
 	auto cleanup = [&](int rc) { hydro_destroy(h); return rc; };
 	if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -233,8 +235,11 @@ extern "C" int hydro_create(const hydro_cfg *cfg, hydro_solver **out) {
 	}
 	if (o == 1) {
 		if (hipMalloc(&h->ping, stateBytes) != hipSuccess) return cleanup(fail("hipMalloc ping"));
+		if (hipMemsetAsync(h->ping, 0, stateBytes, h->stream) != hipSuccess) return cleanup(fail("clear ping"));
 		h->devBytes += stateBytes;
 	}
+	for (int i = 0; i < o; ++i)
+		if (h->Dbuf[i] && hipMemsetAsync(h->Dbuf[i], 0, stateBytes, h->stream) != hipSuccess) return cleanup(fail("clear deriv"));
 	if (h->NSTAT) {
 		if (hipMalloc(&h->stat, planeBytes * h->NSTAT) != hipSuccess) return cleanup(fail("hipMalloc static"));
 		h->devBytes += planeBytes * h->NSTAT;
@@ -245,7 +250,8 @@ extern "C" int hydro_create(const hydro_cfg *cfg, hydro_solver **out) {
 	if (hipMalloc((void **)&h->totalsDev, sizeof(double) * HYDRO_MAX_STATES) != hipSuccess) return cleanup(fail("hipMalloc totals"));
 	DevScalars z;
 	std::memset(&z, 0, sizeof z);
-	z.dtmin[0] = z.dtmin[1] = z.dtmin[2] = INFINITY;
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < DT_LANES; ++j) z.dtmin[i][j] = INFINITY;
 	if (hipMemcpyAsync(h->dev, &z, sizeof z, hipMemcpyHostToDevice, h->stream) != hipSuccess) return cleanup(fail("init scalars"));
 	for (int i = 0; i < o; ++i)
 		if (hipMemsetAsync(h->Ubuf[i], 0, stateBytes, h->stream) != hipSuccess) return cleanup(fail("clear state"));
@@ -278,6 +284,21 @@ extern "C" size_t hydro_num_cells(const hydro_solver *h) {
 }
 
 extern "C" size_t hydro_device_bytes(const hydro_solver *h) { return h ? h->devBytes : 0; }
+
+extern "C" int hydro_set_launch(hydro_solver *h, int rows_per_wave, int xcd_remap) {
+	if (!h) return fail("NULL handle");
+	if (rows_per_wave < 0 || rows_per_wave > 4096) return fail("rows_per_wave %d out of range", rows_per_wave);
+	if (rows_per_wave > 0 && h->dim >= 2) h->rowsPerWave = rows_per_wave;
+	if (xcd_remap > 0) h->xcdRemap = xcd_remap == 1;
+	return 0;
+}
+
+extern "C" int hydro_get_launch(const hydro_solver *h, int *rows_per_wave, int *xcd_remap) {
+	if (!h) return fail("NULL handle");
+	if (rows_per_wave) *rows_per_wave = h->rowsPerWave;
+	if (xcd_remap) *xcd_remap = h->xcdRemap ? 1 : 2;
+	return 0;
+}
 
 // ------------------------------------------------------- layout conversion
 // Ghost coordinate -> interior source, following the reference's x, y, z
@@ -373,6 +394,17 @@ template <typename R>
 static int upload_t(hydro_solver *h, const void *U) {
 	std::vector<R> planes, stat;
 	pack_interior<R>(h, (const R *)U, planes, stat, h->ePot);
+	// Euler: momentum components beyond dim that are zero everywhere stay zero
+	// (their fluxes and wave amplitudes are exact zeros), so the kernels can
+	// skip them
+	h->nvel = 3;
+	if (h->cfg.eqn == HYDRO_EQN_EULER && h->dim < 3) {
+		bool zero = true;
+		for (int f = 1 + h->dim; f <= 3 && zero; ++f)
+			for (long c = 0; c < h->ps; ++c)
+				if (planes[(size_t)f * h->ps + c] != R(0)) { zero = false; break; }
+		if (zero) h->nvel = h->dim;
+	}
 	HIPCHECK(hipMemcpyAsync(h->Ubuf[0], planes.data(), planes.size() * sizeof(R), hipMemcpyHostToDevice, h->stream));
 	if (h->NSTAT) HIPCHECK(hipMemcpyAsync(h->stat, stat.data(), stat.size() * sizeof(R), hipMemcpyHostToDevice, h->stream));
 	HIPCHECK(hipStreamSynchronize(h->stream));
@@ -427,16 +459,16 @@ static GridArgs<R> grid_args(hydro_solver *h, int slot) {
 
 static int run_dt_kernel(hydro_solver *h, int slot) {
 	HIPCHECK(launch_reset_slots(h->dev, 1 << slot, h->stream));
-	hipError_t e = h->elem == 8 ? launch_dt<double>(h->cfg.eqn, h->dim, grid_args<double>(h, slot), h->stream)
-	                            : launch_dt<float>(h->cfg.eqn, h->dim, grid_args<float>(h, slot), h->stream);
+	hipError_t e = h->elem == 8 ? launch_dt<double>(h->cfg.eqn, h->dim, h->cfg.fast_math, grid_args<double>(h, slot), h->stream)
+	                            : launch_dt<float>(h->cfg.eqn, h->dim, h->cfg.fast_math, grid_args<float>(h, slot), h->stream);
 	if (e != hipSuccess) return fail("dt kernel: %s", hipGetErrorString(e));
 	return 0;
 }
 
 extern "C" int hydro_constrain_U(hydro_solver *h) {
 	if (!h) return fail("NULL handle");
-	hipError_t e = h->elem == 8 ? launch_constrain<double>(h->cfg.eqn, grid_args<double>(h, 0), h->stream)
-	                            : launch_constrain<float>(h->cfg.eqn, grid_args<float>(h, 0), h->stream);
+	hipError_t e = h->elem == 8 ? launch_constrain<double>(h->cfg.eqn, h->cfg.fast_math, grid_args<double>(h, 0), h->stream)
+	                            : launch_constrain<float>(h->cfg.eqn, h->cfg.fast_math, grid_args<float>(h, 0), h->stream);
 	if (e != hipSuccess) return fail("constrainU kernel: %s", hipGetErrorString(e));
 	h->dtValid = 0;
 	return 0;
@@ -448,8 +480,9 @@ extern "C" int hydro_calc_dt(hydro_solver *h, double *dt) {
 	if (!h || !dt) return fail("NULL argument");
 	if (h->cfg.use_fixed_dt) { *dt = h->cfg.fixed_dt; return 0; }
 	if (run_dt_kernel(h, h->slot)) return -1;
+	HIPCHECK(launch_fold_slot(h->dev, h->slot, h->stream));
 	double m;
-	HIPCHECK(hipMemcpyAsync(&m, &h->dev->dtmin[h->slot], sizeof m, hipMemcpyDeviceToHost, h->stream));
+	HIPCHECK(hipMemcpyAsync(&m, &h->dev->dtmin[h->slot][0], sizeof m, hipMemcpyDeviceToHost, h->stream));
 	HIPCHECK(hipStreamSynchronize(h->stream));
 	h->dtValid = 1;
 	*dt = h->cfg.cfl * m;
@@ -500,6 +533,7 @@ static void stage_args(hydro_solver *h, int s, StageArgs<R> &a, void *out) {
 	for (int i = 0; i < 3; ++i) {
 		dxr[i] = R((c.maxs[i] - c.mins[i]) / (double)h->n[i]);
 		a.dx[i] = dxr[i];
+		a.invdx[i] = R(1) / dxr[i];
 		a.dxD[i] = (c.maxs[i] - c.mins[i]) / (double)h->n[i];
 	}
 	R volume = R(1);
@@ -547,6 +581,8 @@ static StageKey stage_key(const hydro_solver *h) {
 	k.plm = h->cfg.plm != HYDRO_PLM_NONE;
 	k.lim = !k.plm && h->cfg.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
 	k.limiter = k.plm ? h->cfg.slope_limiter : h->cfg.flux_limiter;
+	k.fast = h->cfg.fast_math != 0;
+	k.nvel = h->nvel;
 	return k;
 }
 

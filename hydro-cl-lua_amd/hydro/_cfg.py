@@ -60,6 +60,7 @@ class HydroCfg(ctypes.Structure):
         ('P_min', ctypes.c_double),
         ('div_phi_wavespeed', ctypes.c_double),
         ('div_psi_wavespeed', ctypes.c_double),
+        ('fast_math', ctypes.c_int),
     ]
 
 
@@ -79,7 +80,7 @@ def make_cfg(dim, size, mins=None, maxs=None, eqn='euler', flux='roe',
              integrator='forward Euler', cfl=0.5, fixed_dt=None,
              boundary='freeflow', boundary_fixed=None, precision='double',
              gamma=7 / 5, rho_min=1e-7, P_min=1e-7,
-             div_phi_wavespeed=1.0, div_psi_wavespeed=1.0):
+             div_phi_wavespeed=1.0, div_psi_wavespeed=1.0, fast_math=False):
     """Builds a HydroCfg from reference-style arguments (names or indices).
 
     `boundary` is a name for all faces or a dict face -> name, like the
@@ -117,6 +118,7 @@ def make_cfg(dim, size, mins=None, maxs=None, eqn='euler', flux='roe',
     c.P_min = float(P_min)
     c.div_phi_wavespeed = float(div_phi_wavespeed)
     c.div_psi_wavespeed = float(div_psi_wavespeed)
+    c.fast_math = 1 if fast_math else 0
     return c
 
 

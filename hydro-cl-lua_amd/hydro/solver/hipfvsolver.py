@@ -49,7 +49,7 @@ class HipFVSolver:
     def __init__(self, dim, gridSize, eqn='euler', flux='roe', integrator='forward Euler',
                  fluxLimiter='donor cell', usePLM=None, slopeLimiter='minmod', cfl=.5,
                  fixedDT=None, mins=None, maxs=None, boundary='freeflow', boundaryFixed=None,
-                 initCond=None, initCondArgs=None, real='double', eqnArgs=None):
+                 initCond=None, initCondArgs=None, real='double', eqnArgs=None, fastMath=False):
         eqnArgs = dict(eqnArgs or {})
         self.cfg = make_cfg(dim=dim, size=gridSize, mins=mins, maxs=maxs, eqn=eqn, flux=flux,
                             flux_limiter=fluxLimiter, plm=usePLM, slope_limiter=slopeLimiter,
@@ -58,7 +58,8 @@ class HipFVSolver:
                             gamma=eqnArgs.get('heatCapacityRatio', 7 / 5),
                             rho_min=eqnArgs.get('rhoMin', 1e-7), P_min=eqnArgs.get('PMin', 1e-7),
                             div_phi_wavespeed=eqnArgs.get('divPhiWavespeed', 1.),
-                            div_psi_wavespeed=eqnArgs.get('divPsiWavespeed', 1.))
+                            div_psi_wavespeed=eqnArgs.get('divPsiWavespeed', 1.),
+                            fast_math=fastMath)
         self.dim = int(dim)
         self.dtype = np.float64 if real == 'double' else np.float32
         self.shape = cfg_ghost_shape(self.cfg)
@@ -156,6 +157,15 @@ class HipFVSolver:
         t = _abi.HydroTiming()
         _abi.check(_abi.lib().hydro_time_updates(self.h, int(n), 1 if perKernel else 0, ctypes.byref(t)))
         return t
+
+    def setLaunch(self, rowsPerWave=0, xcdRemap=0):
+        """launch shape of the stage kernel (xcdRemap: 1 on, 2 off, 0 keep)"""
+        _abi.check(_abi.lib().hydro_set_launch(self.h, int(rowsPerWave), int(xcdRemap)))
+
+    def getLaunch(self):
+        r, x = ctypes.c_int(), ctypes.c_int()
+        _abi.check(_abi.lib().hydro_get_launch(self.h, ctypes.byref(r), ctypes.byref(x)))
+        return r.value, x.value
 
     def interior(self, U):
         sl = [slice(None)] * 4

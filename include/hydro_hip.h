@@ -115,6 +115,9 @@ typedef struct hydro_cfg {
 	double P_min;            /* Euler PMin (euler.lua:196) */
 	double div_phi_wavespeed;/* GLM-Maxwell divPhiWavespeed (glm-maxwell.lua:110) */
 	double div_psi_wavespeed;/* GLM-Maxwell divPsiWavespeed (glm-maxwell.lua:111) */
+	int fast_math;           /* 0: IEEE division and sqrt, the reference's arithmetic;
+	                          * 1: faithfully rounded hardware sequences (FP64: rcp +
+	                          * Newton + residual correction; FP32: v_rcp / v_sqrt) */
 } hydro_cfg;
 
 /* Reference defaults: mins -1, maxs 1 (gridsolver.lua:80-81), gamma 7/5,
@@ -195,6 +198,14 @@ typedef struct hydro_timing {
 	int n_dt_launches;
 } hydro_timing;
 int hydro_time_updates(hydro_solver *h, int nsteps, int per_kernel_events, hydro_timing *out);
+
+/* Launch shape of the fused stage kernel (no counterpart in the reference,
+ * whose work sizes are fixed at gridsolver.lua:141-243): rows each 64-lane
+ * wave marches in y, and whether blocks are remapped so that each XCD works on
+ * a contiguous band of strips. 0 keeps the current value. The autotuner sets
+ * these; results do not depend on them. */
+int hydro_set_launch(hydro_solver *h, int rows_per_wave, int xcd_remap);
+int hydro_get_launch(const hydro_solver *h, int *rows_per_wave, int *xcd_remap);
 
 /* Bytes of the device state held by the handle. */
 size_t hydro_device_bytes(const hydro_solver *h);

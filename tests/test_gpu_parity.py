@@ -30,7 +30,7 @@ def _pair(init, nsteps, **kw):
 
 
 def _oracle_kw(kw):
-    m = {'dim': 'dim', 'gridSize': 'size', 'mins': 'mins', 'maxs': 'maxs', 'usePLM': 'plm',
+    m = {'dim': 'dim', 'gridSize': 'size', 'mins': 'mins', 'maxs': 'maxs', 'usePLM': 'plm', 'eqn': 'eqn',
          'slopeLimiter': 'slope_limiter', 'fluxLimiter': 'flux_limiter', 'integrator': 'integrator',
          'cfl': 'cfl', 'boundary': 'boundary', 'fixedDT': 'fixed_dt', 'boundaryFixed': 'boundary_fixed'}
     out = {m[k]: v for k, v in kw.items() if k in m}
@@ -169,3 +169,99 @@ def test_kh_fp32_vs_fp64_oracle():
               real='float')
     s, o = _pair(_kh(0), 50, **kw)
     _check(s, o, 2, TOL32, 50, 'kh-fp32')
+
+
+def _glm(seed, sigma=0.):
+    from hydro.init import glm_maxwell
+    return lambda cfg: glm_maxwell.gaussian_pulses(cfg, seed=seed, conductivity=sigma)
+
+
+@pytest.mark.parametrize('n,plm,bc,sigma', [((64, 48), 'plm cons', 'periodic', 0.), ((50, 40), None, 'freeflow', .5),
+                                             ((40, 33), 'plm cons', 'mirror', 1.)])
+def test_glm_maxwell_2d(n, plm, bc, sigma):
+    kw = dict(dim=2, gridSize=n, eqn='glm_maxwell', usePLM=plm, slopeLimiter='minmod',
+              integrator='Runge-Kutta 2, TVD', cfl=.4, boundary=bc)
+    s, o = _pair(_glm(7, sigma), 20, **kw)
+    s.update(20)
+    o.update(20)
+    got = _interior(2, s.UBufObj.toCPU())
+    ref = _interior(2, o.U)
+    err = rel_l1(got[..., :8], ref[..., :8])
+    print('glm', n, err)
+    assert np.all(err <= TOL64), err
+    assert np.array_equal(got[..., 8:], ref[..., 8:])   # static fields carried unchanged
+
+
+def test_glm_maxwell_1d_flux_limiter():
+    kw = dict(dim=1, gridSize=300, eqn='glm_maxwell', fluxLimiter='superbee',
+              integrator='Runge-Kutta 3, TVD', cfl=.4, boundary='freeflow')
+    s, o = _pair(_glm(8), 30, **kw)
+    s.update(30)
+    o.update(30)
+    err = rel_l1(_interior(1, s.UBufObj.toCPU())[..., :8], _interior(1, o.U)[..., :8])
+    assert np.all(err <= TOL64), err
+
+
+# ---- FAST math policy: faithfully rounded division / sqrt, still <= 1e-12
+@pytest.mark.parametrize('n,steps', [((128, 128), 100), ((96, 70), 60)])
+def test_kh_fast_math_fp64(n, steps):
+    kw = dict(dim=2, gridSize=n, mins=(-.5, -.5), maxs=(.5, .5), usePLM='plm cons', slopeLimiter='minmod',
+              integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic', fastMath=True)
+    s, o = _pair(_kh(4), steps, **kw)
+    _check(s, o, 2, TOL64, steps, f'kh-fast{n}')
+
+
+@pytest.mark.parametrize('lim,integ', [('superbee', 'forward Euler'), ('van Leer', 'Runge-Kutta 3, TVD'),
+                                       ('Koren', 'Runge-Kutta 2, TVD')])
+def test_flux_limiter_fast_math(lim, integ):
+    kw = dict(dim=2, gridSize=(48, 52), fluxLimiter=lim, integrator=integ, cfl=.3, boundary='freeflow',
+              fastMath=True)
+    s, o = _pair(_quad(9), 20, **kw)
+    _check(s, o, 2, TOL64, 20, f'fast-{lim}')
+
+
+def test_sod_fast_math():
+    left = [1., 0, 0, 0, 2.5, 0]
+    right = [.125, 0, 0, 0, .25, 0]
+    kw = dict(dim=1, gridSize=256, fluxLimiter='superbee', integrator='forward Euler', cfl=.5,
+              boundary={'xmin': 'fixed', 'xmax': 'fixed'}, boundaryFixed={'xmin': left, 'xmax': right},
+              fastMath=True)
+    s, o = _pair(_sod, 100, **kw)
+    _check(s, o, 1, TOL64, 100, 'sod-fast')
+
+
+def test_glm_fast_math():
+    kw = dict(dim=2, gridSize=(64, 64), eqn='glm_maxwell', usePLM='plm cons', slopeLimiter='minmod',
+              integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic', fastMath=True)
+    s, o = _pair(_glm(11), 30, **kw)
+    s.update(30)
+    o.update(30)
+    err = rel_l1(_interior(2, s.UBufObj.toCPU())[..., :8], _interior(2, o.U)[..., :8])
+    assert np.all(err <= TOL64), err
+
+
+def test_nonzero_out_of_plane_momentum_2d():
+    """a 2D state with m.z != 0 takes the three-component kernels"""
+    def init(cfg):
+        U = _kh(5)(cfg)
+        U[..., 3] = .1 * U[..., 0]
+        U[..., 4] += .5 * .01 * U[..., 0]
+        return U
+    kw = dict(dim=2, gridSize=(64, 40), mins=(-.5, -.5), maxs=(.5, .5), usePLM='plm cons', slopeLimiter='minmod',
+              integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic')
+    for fast in (False, True):
+        s, o = _pair(init, 20, fastMath=fast, **kw)
+        s.update(20)
+        o.update(20)
+        got = _interior(2, s.UBufObj.toCPU())[..., :5]
+        err = rel_l1(got, _interior(2, o.U)[..., :5])
+        assert np.all(err <= TOL64), (fast, err)
+        assert np.abs(got[..., 3]).max() > 0
+
+
+def test_kh_fp32_fast_vs_fp64_oracle():
+    kw = dict(dim=2, gridSize=(128, 128), mins=(-.5, -.5), maxs=(.5, .5), usePLM='plm cons',
+              slopeLimiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic',
+              real='float', fastMath=True)
+    s, o = _pair(_kh(0), 50, **kw)
+    _check(s, o, 2, TOL32, 50, 'kh-fp32-fast')

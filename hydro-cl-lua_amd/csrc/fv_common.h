@@ -30,47 +30,69 @@ struct AxisBC {
 	int kind[2];      // HYDRO_BC_* at min / max
 };
 
+// Everything that is specific to one grid: its buffers, shape, boundaries,
+// metric factors, launch decomposition and device scalars. The plain solver
+// passes one by value; a batch keeps an array of them in device memory.
 template <typename R>
-struct StageArgs {
-	// state: q integrated planes, each nx*ny*nz (x fastest), plane stride ps
-	const R *src;             // U of the stage whose derivative is taken (halo reads)
-	R *out;                   // new stage state (interior writes)
+struct GridDesc {
+	R *U[4];                  // stage states U^(0..order-1); U[0] holds the solver state
+	R *D[4];                  // stored stage derivatives (tables whose later rows use them)
+	R *ping;                  // forward-Euler output (swapped with U[0] after a step)
 	const R *stat;            // static (non-integrated) planes, GLM-Maxwell only
-	const R *accU[4];         // alpha terms, reference order
-	double alpha[4];
-	int nAlpha;
-	const R *accD[3];         // stored derivatives of earlier stages (beta terms)
-	double beta[3];
-	int nStoredD;
-	R *storeD;                // store this stage's derivative (if later stages use it)
-	double betaNew;           // beta of this stage's derivative (0: not used)
-	int n[3];
+	int n[3];                 // interior cells
 	long ps;                  // plane stride (reals)
-	long step[3];             // 1, nx, nx*ny
-	int bc[6];
-	R fixedv[6][HYDRO_MAX_STATES];   // HYDRO_BC_FIXED ghost values (integrated then static fields)
+	long step1;               // row stride (= n[0])
+	int bc[6];                // HYDRO_BC_* per face
+	R fixedv[6][HYDRO_MAX_STATES];   // HYDRO_BC_FIXED ghost values
 	R areaOverVol[3];         // areaL * (1 / volume) per side (fvsolver.cl:64-102)
 	int derivOn;              // volume > 1e-7 (fvsolver.cl:97)
 	int fluxOn[3];            // cell_area > 1e-7 (fvsolver.lua:105)
 	R dx[3];
 	R invdx[3];               // 1 / dx in the stored precision (glm-maxwell.cl:351)
 	double dxD[3];
+	double cfl;
+	int stripsX, stripsY, rowsPerWave;
+	int active;               // batch: grid takes part in the launch
+	long itemBase;            // batch: first work item (wave strip) of this grid
+	long nItems;              // stripsX * stripsY
+	DevScalars *dev;          // dt slots and time of this grid
+	int stepsLeft;            // batch: steps still to run in the current tick
+	int pad;
+};
+
+// Uniform for one launch: the stage's row of the Shu-Osher table (as buffer
+// indices), the dt protocol and the equation parameters.
+template <typename R>
+struct StageArgs {
+	int src;                  // buffer index of the stage whose derivative is taken
+	int out;                  // output buffer index, -1: ping
+	int accU[4];              // alpha terms, reference order
+	double alpha[4];
+	int nAlpha;
+	int accD[3];              // stored derivatives of earlier stages (beta terms)
+	double beta[3];
+	int nStoredD;
+	int storeD;               // store this stage's derivative into D[storeD] (-1: no)
+	double betaNew;           // beta of this stage's derivative (0: not used)
 	// dt: device slot (cfl * dtmin[slot]) or a host value
-	DevScalars *dev;
 	int dtSlot;               // -1: use hostDt
 	double hostDt;
-	double cfl;
-	int resetSlot;            // slot to clear to +inf (block 0), -1 none
+	int resetSlot;            // slot to clear to +inf, -1 none
 	int reduceSlot;           // slot to min-reduce the new state's dt into, -1 none
-	int advanceTime;          // block 0: t += dt, dt stored
+	int advanceTime;          // t += dt, dt stored
+	int stepInTick;           // batch: skip grids with stepsLeft <= stepInTick
 	// equation parameters in the stencil type and in double (for the FP64
 	// accumulation, constrainU and dt work of the FP32 form)
 	R gamma, rhoMin, PMin, divPhi, divPsi;
 	double gammaD, rhoMinD, PMinD, divPhiD, divPsiD;
 	int limiter;              // flux or slope limiter index (runtime switch form)
-	int stripsX, stripsY, rowsPerWave;
-	int nBlocks;
 	int xcdRemap;
+	int nBlocks;
+	// one grid (plain launch) or a batch (persistent launch)
+	GridDesc<R> g;
+	const GridDesc<R> *grids;
+	int nGrids;
+	long totalItems;
 };
 
 // ---------------------------------------------------------------- lane moves

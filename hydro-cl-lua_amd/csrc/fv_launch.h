@@ -34,6 +34,8 @@ inline int limiter_template_id(int index) {
 
 template <typename R> struct GridArgs;
 
+// grid.x > 0: plain launch of that many blocks; grid.x == 0: persistent
+// batched launch over a.grids.
 template <typename R>
 hipError_t launch_stage(const StageKey &k, const StageArgs<R> &a, dim3 grid, dim3 block, hipStream_t s);
 

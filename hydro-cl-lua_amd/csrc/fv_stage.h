@@ -30,10 +30,10 @@ struct Cell {
 
 // ---------------------------------------------------------- state loading
 template <typename R, class E>
-__device__ __forceinline__ void load_cell(const StageArgs<R> &a, const R *base, const R *stat,
-	const Mapped &mx, const Mapped &my, Cell<R, E> &c)
+__device__ __forceinline__ void load_cell(const GridDesc<R> &g, const R *base, const Mapped &mx, const Mapped &my,
+	Cell<R, E> &c)
 {
-	long off = (long)my.idx * a.step[1] + mx.idx;
+	long off = (long)my.idx * g.step1 + mx.idx;
 #pragma unroll
 	for (int f = 0; f < Cell<R, E>::NV; ++f) {
 		if (!E::active(f)) {
@@ -41,11 +41,11 @@ __device__ __forceinline__ void load_cell(const StageArgs<R> &a, const R *base, 
 			continue;
 		}
 		R v;
-		if (f < E::Q) v = base[f * a.ps + off];
-		else v = stat[(f - E::Q) * a.ps + off];
-		if (mx.fixedFace >= 0) v = a.fixedv[mx.fixedFace][f];
+		if (f < E::Q) v = base[f * g.ps + off];
+		else v = g.stat[(f - E::Q) * g.ps + off];
+		if (mx.fixedFace >= 0) v = g.fixedv[mx.fixedFace][f];
 		else if (mx.mirror && E::reflects(f, 0)) v = -v;
-		if (my.fixedFace >= 0) v = a.fixedv[my.fixedFace][f];
+		if (my.fixedFace >= 0) v = g.fixedv[my.fixedFace][f];
 		else if (my.mirror && E::reflects(f, 1)) v = -v;
 		c.v[f] = v;
 	}
@@ -145,9 +145,13 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 	for (int j = 0; j < E::Q; ++j) F[j] = Y[j] + I.Favg[j];
 }
 
-// ---------------------------------------------------------- the kernel
+// ------------------------------------------------------- one wave strip
+// Advances the strip (sx, sy) of grid g by one stage: 60 output columns
+// (lanes 2..61) by rowsPerWave rows. Returns the strip's dt minimum of the new
+// state when the stage reduces dt.
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
-__global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
+__device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridDesc<R> &g, int sx, int sy, double dt,
+	int lane)
 {
 	using E = EQ<R>;
 	using C = Cell<R, E>;
@@ -155,217 +159,283 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 	const typename E::Par par = E::par(a);
 	const int lim = PLM ? 0 : a.limiter;   // runtime limiter index when FLIM/SLIM == -1
 	const int slim = a.limiter;
-
-	const int lane = threadIdx.x & 63;
-	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
-	const int wave = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
-
-	// dt of this step (solverbase.lua:3016): cfl * min, or a host value
-	double dt = a.dtSlot >= 0 ? a.cfl * read_dt_slot(a.dev, a.dtSlot) : a.hostDt;
-	if (blockIdx.x == 0 && threadIdx.x < DT_LANES && a.resetSlot >= 0) a.dev->dtmin[a.resetSlot][threadIdx.x] = INFINITY;
-
-	const int nx = a.n[0];
-	const int ny = DIM >= 2 ? a.n[1] : 1;
-	const int sx = wave % a.stripsX;
-	const int sy = wave / a.stripsX;
 	double dtLocal = INFINITY;
 
-	if (sy < a.stripsY) {
-		const int x = sx * STRIP_OUT - NG + lane;
-		const bool outLane = lane >= NG && lane < WAVE - NG && x < nx;
-		const Mapped mx = map_coord(x, nx, a.bc[0], a.bc[1], 0);
-		Mapped myIn;
-		myIn.idx = 0; myIn.mirror = 0; myIn.fixedFace = -1;
-		const R rdt = R(dt);
-		const R dt_dx0 = rdt / a.dx[0];
-		const R dt_dx1 = DIM >= 2 ? rdt / a.dx[1] : R(0);
-		const int y0 = sy * a.rowsPerWave;
-		const int y1 = min(y0 + a.rowsPerWave, ny);
+	const int nx = g.n[0];
+	const int ny = DIM >= 2 ? g.n[1] : 1;
+	const R *src = g.U[a.src];
+	R *out = a.out >= 0 ? g.U[a.out] : g.ping;
+	R *storeD = a.storeD >= 0 ? g.D[a.storeD] : nullptr;
+	const int x = sx * STRIP_OUT - NG + lane;
+	const bool outLane = lane >= NG && lane < WAVE - NG && x < nx;
+	const Mapped mx = map_coord(x, nx, g.bc[0], g.bc[1], 0);
+	Mapped myIn;
+	myIn.idx = 0; myIn.mirror = 0; myIn.fixedFace = -1;
+	const R rdt = R(dt);
+	const R dt_dx0 = rdt / g.dx[0];
+	const R dt_dx1 = DIM >= 2 ? rdt / g.dx[1] : R(0);
+	const int y0 = sy * g.rowsPerWave;
+	const int y1 = min(y0 + g.rowsPerWave, ny);
 
-		// y pipeline state
-		C Wm, W0, W1, W2;       // rows r-1 .. r+2
-		C Ry0;                  // PLM: R edge of row r (from the previous iteration)
-		Iface<R, E> I0, I1;     // LIM: interfaces r-1/2 and r+1/2
-		R FaYprev[Q];
+	// y pipeline state
+	C Wm, W0, W1, W2;       // rows r-1 .. r+2
+	C Ry0;                  // PLM: R edge of row r (from the previous iteration)
+	Iface<R, E> I0, I1;     // LIM: interfaces r-1/2 and r+1/2
+	R FaYprev[Q];
 #pragma unroll
-		for (int f = 0; f < C::NV; ++f) { Wm.v[f] = W0.v[f] = W1.v[f] = W2.v[f] = Ry0.v[f] = R(0); }
+	for (int f = 0; f < C::NV; ++f) { Wm.v[f] = W0.v[f] = W1.v[f] = W2.v[f] = Ry0.v[f] = R(0); }
 #pragma unroll
-		for (int j = 0; j < Q; ++j) FaYprev[j] = R(0);
-		I0 = Iface<R, E>{};
-		I1 = Iface<R, E>{};
+	for (int j = 0; j < Q; ++j) FaYprev[j] = R(0);
+	I0 = Iface<R, E>{};
+	I1 = Iface<R, E>{};
 
-		auto loadRow = [&](int r, C &c) {
-			Mapped my = DIM >= 2 ? map_coord(r, ny, a.bc[2], a.bc[3], 2) : myIn;
-			load_cell<R, E>(a, a.src, a.stat, mx, my, c);
-		};
+	auto loadRow = [&](int r, C &c) {
+		Mapped my = DIM >= 2 ? map_coord(r, ny, g.bc[2], g.bc[3], 2) : myIn;
+		load_cell<R, E>(g, src, mx, my, c);
+	};
 
-		// iterations before y0 only fill the pipeline
-		constexpr int WARM = DIM < 2 ? 0 : (PLM ? 2 : (LIM ? 3 : 1));
+	// iterations before y0 only fill the pipeline
+	constexpr int WARM = DIM < 2 ? 0 : (PLM ? 2 : (LIM ? 3 : 1));
+	if constexpr (DIM >= 2) {
+		if constexpr (PLM) { loadRow(y0 - 2, W0); loadRow(y0 - 1, W1); }
+		else if constexpr (LIM) { loadRow(y0 - 2, W1); }
+		else { loadRow(y0 - 1, W0); }
+	}
+
+	for (int r = y0 - WARM; r < y1; ++r) {
+		// ---- y direction: interface r+1/2, dyterm of row r
+		R dyterm[Q];
 		if constexpr (DIM >= 2) {
-			if constexpr (PLM) { loadRow(y0 - 2, W0); loadRow(y0 - 1, W1); }
-			else if constexpr (LIM) { loadRow(y0 - 2, W1); }
-			else { loadRow(y0 - 1, W0); }
-		}
-
-		for (int r = y0 - WARM; r < y1; ++r) {
-			// ---- y direction: W0 = U(r) after this block
-			R dyterm[Q];
-			if constexpr (DIM >= 2) {
-				R FaY[Q];
-				R F[Q];
-				if constexpr (PLM) {
-					// W0 = U(r), W1 = U(r+1): cell r+1's edges, interface r+1/2
-					loadRow(r + 2, W2);
-					C L1, R1;
-					plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
+			R F[Q];
+			// the first PLM warm-up iteration's interface is never used
+			const bool wantFlux = !(PLM && r == y0 - WARM);
+			if constexpr (PLM) {
+				// W0 = U(r), W1 = U(r+1): cell r+1's edges, interface r+1/2
+				loadRow(r + 2, W2);
+				C L1, R1;
+				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
+				if (wantFlux) {
 					Iface<R, E> I;
 					iface_prep<R, E, 1>(par, Ry0, L1, I);
 					iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, F);
-					Ry0 = R1;
-				} else if constexpr (LIM) {
-					// W1 = U(r+1): interface r+3/2, flux at r+1/2 from its neighbours
-					loadRow(r + 2, W2);
-					Iface<R, E> I2;
-					iface_prep<R, E, 1>(par, W1, W2, I2);
-					iface_flux<R, E, 1, true, FLIM>(par, I1, I0.dW, I2.dW, dt_dx1, lim, F);
-					I0 = I1;
-					I1 = I2;
 				} else {
-					// W0 = U(r): interface r+1/2
-					loadRow(r + 1, W1);
-					Iface<R, E> I;
-					iface_prep<R, E, 1>(par, W0, W1, I);
-					iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, F);
-				}
 #pragma unroll
-				for (int j = 0; j < Q; ++j) {
-					FaY[j] = a.fluxOn[1] ? F[j] * a.areaOverVol[1] : R(0);
-					dyterm[j] = FaY[j] - FaYprev[j];
-					FaYprev[j] = FaY[j];
+					for (int j = 0; j < Q; ++j) F[j] = R(0);
 				}
+				Ry0 = R1;
+			} else if constexpr (LIM) {
+				// W1 = U(r+1): interface r+3/2, flux at r+1/2 from its neighbours
+				loadRow(r + 2, W2);
+				Iface<R, E> I2;
+				iface_prep<R, E, 1>(par, W1, W2, I2);
+				iface_flux<R, E, 1, true, FLIM>(par, I1, I0.dW, I2.dW, dt_dx1, lim, F);
+				I0 = I1;
+				I1 = I2;
+			} else {
+				// W0 = U(r): interface r+1/2
+				loadRow(r + 1, W1);
+				Iface<R, E> I;
+				iface_prep<R, E, 1>(par, W0, W1, I);
+				iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, F);
 			}
-
-			if (r >= y0) {
-				// ---- x direction on row r (lanes)
-				C U;
-				if constexpr (DIM >= 2) U = W0;
-				else loadRow(0, U);
-				R Fx[Q];
-				if constexpr (PLM) {
-					C Um, Up, Lx, Rx, RxL;
-					shift_left<R, E>(U, Um);
-					shift_right<R, E>(U, Up);
-					plm_cons<R, E, SLIM>(slim, Um, U, Up, Lx, Rx);
-					shift_left<R, E>(Rx, RxL);
-					Iface<R, E> I;
-					iface_prep<R, E, 0>(par, RxL, Lx, I);
-					iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
-				} else if constexpr (LIM) {
-					C Um;
-					shift_left<R, E>(U, Um);
-					Iface<R, E> I;
-					iface_prep<R, E, 0>(par, Um, U, I);
-					R dWL[E::NW], dWR[E::NW];
 #pragma unroll
-					for (int j = 0; j < E::NW; ++j) { dWL[j] = from_left(I.dW[j]); dWR[j] = from_right(I.dW[j]); }
-					iface_flux<R, E, 0, true, FLIM>(par, I, dWL, dWR, dt_dx0, lim, Fx);
-				} else {
-					C Um;
-					shift_left<R, E>(U, Um);
-					Iface<R, E> I;
-					iface_prep<R, E, 0>(par, Um, U, I);
-					iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
-				}
-				R deriv[Q];
-#pragma unroll
-				for (int j = 0; j < Q; ++j) {
-					R Fa = a.fluxOn[0] ? Fx[j] * a.areaOverVol[0] : R(0);
-					R FaR = from_right(Fa);
-					deriv[j] = R(0) - (FaR - Fa);
-					if constexpr (DIM >= 2) deriv[j] = deriv[j] - dyterm[j];
-					if (!a.derivOn) deriv[j] = R(0);
-				}
-				if constexpr (E::SOURCE) {
-					// eqn addSource after calcDerivFromFlux (solverbase.lua:3208-3214)
-					R em[3], ep[3];
-					em[0] = from_left(U.v[10]);
-					ep[0] = from_right(U.v[10]);
-					if constexpr (DIM >= 2) { em[1] = Wm.v[10]; ep[1] = W1.v[10]; }
-					E::template source<DIM>(par, a.invdx, U.v, em, ep, deriv);
-				}
-
-				// ---- integrator combine (rk.lua:155-174, fe.lua:50), constrainU
-				if (outLane) {
-					long off = (long)r * a.step[1] + x;
-					double acc[Q];
-#pragma unroll
-					for (int j = 0; j < Q; ++j) acc[j] = 0.;
-					for (int k = 0; k < a.nAlpha; ++k) {
-						const R *Uk = a.accU[k];
-						double al = a.alpha[k];
-						bool fromSrc = Uk == a.src;
-#pragma unroll
-						for (int j = 0; j < Q; ++j) {
-							if (!E::active(j)) continue;
-							R u = fromSrc ? U.v[j] : Uk[j * a.ps + off];
-							acc[j] = acc[j] + (double)u * al;
-						}
-					}
-					for (int k = 0; k < a.nStoredD; ++k) {
-						const R *Dk = a.accD[k];
-						double c = (double)R(a.beta[k] * dt);
-#pragma unroll
-						for (int j = 0; j < Q; ++j)
-							if (E::active(j)) acc[j] = acc[j] + (double)Dk[j * a.ps + off] * c;
-					}
-					if (a.storeD) {
-#pragma unroll
-						for (int j = 0; j < Q; ++j)
-							if (E::active(j)) a.storeD[j * a.ps + off] = deriv[j];
-					}
-					if (a.betaNew != 0.) {
-						double c = (double)R(a.betaNew * dt);
-#pragma unroll
-						for (int j = 0; j < Q; ++j)
-							if (E::active(j)) acc[j] = acc[j] + (double)deriv[j] * c;
-					}
-					E::constrain(a.gammaD, a.rhoMinD, a.PMinD, acc);
-					R o[Q];
-#pragma unroll
-					for (int j = 0; j < Q; ++j) {
-						o[j] = R(acc[j]);
-						if (E::active(j)) a.out[j * a.ps + off] = o[j];
-					}
-					if (a.reduceSlot >= 0) {
-						double ud[C::NV];
-						const double *dxd = a.dxD;
-#pragma unroll
-						for (int j = 0; j < Q; ++j) ud[j] = (double)o[j];
-#pragma unroll
-						for (int j = Q; j < C::NV; ++j) ud[j] = (double)U.v[j];
-						double d = E::template dt_cell<DIM>(a.gammaD, a.rhoMinD,
-							a.PMinD, dxd, ud, a.divPhiD, a.divPsiD);
-						dtLocal = fmin(dtLocal, d);
-					}
-				}
-			}
-
-			// ---- advance the row window
-			if constexpr (DIM >= 2) {
-				if constexpr (E::SOURCE) Wm = W0;
-				if constexpr (PLM || LIM) { W0 = W1; W1 = W2; }
-				else { W0 = W1; }
+			for (int j = 0; j < Q; ++j) {
+				R FaY = g.fluxOn[1] ? F[j] * g.areaOverVol[1] : R(0);
+				dyterm[j] = FaY - FaYprev[j];
+				FaYprev[j] = FaY;
 			}
 		}
-	}
 
+		if (r >= y0) {
+			// ---- x direction on row r (lanes)
+			C U;
+			if constexpr (DIM >= 2) U = W0;
+			else loadRow(0, U);
+			R Fx[Q];
+			if constexpr (PLM) {
+				C Um, Up, Lx, Rx, RxL;
+				shift_left<R, E>(U, Um);
+				shift_right<R, E>(U, Up);
+				plm_cons<R, E, SLIM>(slim, Um, U, Up, Lx, Rx);
+				shift_left<R, E>(Rx, RxL);
+				Iface<R, E> I;
+				iface_prep<R, E, 0>(par, RxL, Lx, I);
+				iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+			} else if constexpr (LIM) {
+				C Um;
+				shift_left<R, E>(U, Um);
+				Iface<R, E> I;
+				iface_prep<R, E, 0>(par, Um, U, I);
+				R dWL[E::NW], dWR[E::NW];
+#pragma unroll
+				for (int j = 0; j < E::NW; ++j) { dWL[j] = from_left(I.dW[j]); dWR[j] = from_right(I.dW[j]); }
+				iface_flux<R, E, 0, true, FLIM>(par, I, dWL, dWR, dt_dx0, lim, Fx);
+			} else {
+				C Um;
+				shift_left<R, E>(U, Um);
+				Iface<R, E> I;
+				iface_prep<R, E, 0>(par, Um, U, I);
+				iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+			}
+			R deriv[Q];
+#pragma unroll
+			for (int j = 0; j < Q; ++j) {
+				R Fa = g.fluxOn[0] ? Fx[j] * g.areaOverVol[0] : R(0);
+				R FaR = from_right(Fa);
+				deriv[j] = R(0) - (FaR - Fa);
+				if constexpr (DIM >= 2) deriv[j] = deriv[j] - dyterm[j];
+				if (!g.derivOn) deriv[j] = R(0);
+			}
+			if constexpr (E::SOURCE) {
+				// eqn addSource after calcDerivFromFlux (solverbase.lua:3208-3214)
+				R em[3], ep[3];
+				em[0] = from_left(U.v[10]);
+				ep[0] = from_right(U.v[10]);
+				if constexpr (DIM >= 2) { em[1] = Wm.v[10]; ep[1] = W1.v[10]; }
+				E::template source<DIM>(par, g.invdx, U.v, em, ep, deriv);
+			}
+
+			// ---- integrator combine (rk.lua:155-174, fe.lua:50), constrainU
+			if (outLane) {
+				long off = (long)r * g.step1 + x;
+				double acc[Q];
+#pragma unroll
+				for (int j = 0; j < Q; ++j) acc[j] = 0.;
+				for (int k = 0; k < a.nAlpha; ++k) {
+					const bool fromSrc = a.accU[k] == a.src;
+					const R *Uk = g.U[a.accU[k]];
+					double al = a.alpha[k];
+#pragma unroll
+					for (int j = 0; j < Q; ++j) {
+						if (!E::active(j)) continue;
+						R u = fromSrc ? U.v[j] : Uk[j * g.ps + off];
+						acc[j] = acc[j] + (double)u * al;
+					}
+				}
+				for (int k = 0; k < a.nStoredD; ++k) {
+					const R *Dk = g.D[a.accD[k]];
+					double c = (double)R(a.beta[k] * dt);
+#pragma unroll
+					for (int j = 0; j < Q; ++j)
+						if (E::active(j)) acc[j] = acc[j] + (double)Dk[j * g.ps + off] * c;
+				}
+				if (storeD) {
+#pragma unroll
+					for (int j = 0; j < Q; ++j)
+						if (E::active(j)) storeD[j * g.ps + off] = deriv[j];
+				}
+				if (a.betaNew != 0.) {
+					double c = (double)R(a.betaNew * dt);
+#pragma unroll
+					for (int j = 0; j < Q; ++j)
+						if (E::active(j)) acc[j] = acc[j] + (double)deriv[j] * c;
+				}
+				E::constrain(a.gammaD, a.rhoMinD, a.PMinD, acc);
+				R o[Q];
+#pragma unroll
+				for (int j = 0; j < Q; ++j) {
+					o[j] = R(acc[j]);
+					if (E::active(j)) out[j * g.ps + off] = o[j];
+				}
+				if (a.reduceSlot >= 0) {
+					double ud[C::NV];
+#pragma unroll
+					for (int j = 0; j < Q; ++j) ud[j] = (double)o[j];
+#pragma unroll
+					for (int j = Q; j < C::NV; ++j) ud[j] = (double)U.v[j];
+					double d = E::template dt_cell<DIM>(a.gammaD, a.rhoMinD, a.PMinD, g.dxD, ud, a.divPhiD, a.divPsiD);
+					dtLocal = fmin(dtLocal, d);
+				}
+			}
+		}
+
+		// ---- advance the row window
+		if constexpr (DIM >= 2) {
+			if constexpr (E::SOURCE) Wm = W0;
+			if constexpr (PLM || LIM) { W0 = W1; W1 = W2; }
+			else { W0 = W1; }
+		}
+	}
+	return dtLocal;
+}
+
+// dt of this step for grid g (solverbase.lua:3016): cfl * min, or a host value
+template <typename R>
+__device__ __forceinline__ double stage_dt(const StageArgs<R> &a, const GridDesc<R> &g) {
+	return a.dtSlot >= 0 ? g.cfl * read_dt_slot(g.dev, a.dtSlot) : a.hostDt;
+}
+
+// Per-grid bookkeeping done by the wave that owns the grid's first strip:
+// clear the next-but-one dt slot (first stage) and advance t (last stage).
+template <typename R>
+__device__ __forceinline__ void stage_first_strip(const StageArgs<R> &a, const GridDesc<R> &g, double dt, int lane) {
+	if (a.resetSlot >= 0) g.dev->dtmin[a.resetSlot][lane] = INFINITY;
+	if (a.advanceTime && lane == 0) {
+		g.dev->t = g.dev->t + dt;
+		g.dev->dt = dt;
+	}
+}
+
+template <typename R>
+__device__ __forceinline__ void stage_reduce(const StageArgs<R> &a, const GridDesc<R> &g, double dtLocal, long item,
+	int lane)
+{
 	if (a.reduceSlot >= 0) {
 		double m = wave_min(dtLocal);
-		if (lane == 0 && m < INFINITY) atomic_min_pos(&a.dev->dtmin[a.reduceSlot][wave & (DT_LANES - 1)], m);
+		if (lane == 0 && m < INFINITY) atomic_min_pos(&g.dev->dtmin[a.reduceSlot][item & (DT_LANES - 1)], m);
 	}
-	if (a.advanceTime && blockIdx.x == 0 && threadIdx.x == 0) {
-		a.dev->t = a.dev->t + dt;
-		a.dev->dt = dt;
+}
+
+// ---------------------------------------------------------- the kernels
+// Plain form: one grid, one wave per strip.
+template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
+__global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
+{
+	const int lane = threadIdx.x & 63;
+	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
+	const int waveInBlock = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const long item = (long)blk * (blockDim.x >> 6) + waveInBlock;
+	const GridDesc<R> &g = a.g;
+	double dt = stage_dt(a, g);
+	if (item >= g.nItems) return;     // whole waves leave; no barriers follow
+	const int sx = (int)(item % g.stripsX);
+	const int sy = (int)(item / g.stripsX);
+	if (item == 0) stage_first_strip(a, g, dt, lane);
+	double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM>(a, g, sx, sy, dt, lane);
+	stage_reduce(a, g, dtLocal, item, lane);
+}
+
+// Batched form: a persistent launch over every strip of every active grid of
+// the batch; wave k takes items k, k + W, k + 2W, ... (W = waves launched).
+// Items are numbered grid by grid (GridDesc::itemBase), so the owning grid of
+// an item is found by a wave-uniform binary search.
+template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
+__global__ void __launch_bounds__(256) stage_kernel_batch(const StageArgs<R> a)
+{
+	const int lane = threadIdx.x & 63;
+	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
+	const int waveInBlock = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const long wave = (long)blk * (blockDim.x >> 6) + waveInBlock;
+	const long nWaves = (long)gridDim.x * (blockDim.x >> 6);
+	int gi = 0;
+	for (long item = wave; item < a.totalItems; item += nWaves) {
+		// grids are ordered by itemBase; items only increase, so search upward
+		int lo = gi, hi = a.nGrids - 1;
+		while (lo < hi) {
+			int mid = (lo + hi + 1) >> 1;
+			if (a.grids[mid].itemBase <= item) lo = mid;
+			else hi = mid - 1;
+		}
+		gi = lo;
+		const GridDesc<R> &g = a.grids[gi];
+		if (!g.active || g.stepsLeft <= a.stepInTick) continue;
+		const long local = item - g.itemBase;
+		double dt = stage_dt(a, g);
+		const int sx = (int)(local % g.stripsX);
+		const int sy = (int)(local / g.stripsX);
+		if (local == 0) stage_first_strip(a, g, dt, lane);
+		double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM>(a, g, sx, sy, dt, lane);
+		stage_reduce(a, g, dtLocal, local, lane);
 	}
 }
 

@@ -18,6 +18,10 @@ EXPORTS = [
     'hydro_download_U', 'hydro_constrain_U', 'hydro_boundary', 'hydro_calc_dt', 'hydro_step',
     'hydro_update', 'hydro_get_time', 'hydro_set_time', 'hydro_conserved_totals', 'hydro_sync',
     'hydro_time_updates', 'hydro_device_bytes', 'hydro_set_launch', 'hydro_get_launch',
+    'hydro_batch_create', 'hydro_batch_destroy', 'hydro_batch_size', 'hydro_batch_device_bytes',
+    'hydro_batch_upload_U', 'hydro_batch_download_U', 'hydro_batch_constrain_U', 'hydro_batch_update',
+    'hydro_batch_get_time', 'hydro_batch_set_active', 'hydro_batch_sync', 'hydro_batch_time_updates',
+    'hydro_batch_set_launch', 'hydro_batch_get_launch', 'hydro_batch_autotune',
 ]
 
 
@@ -69,6 +73,23 @@ def lib():
     L.hydro_conserved_totals.argtypes = [vp, dp]
     L.hydro_time_updates.argtypes = [vp, ip, ip, ctypes.POINTER(HydroTiming)]
     L.hydro_set_launch.argtypes = [vp, ip, ip]
+    ipp = ctypes.POINTER(ip)
+    L.hydro_batch_create.argtypes = [cfgp, ip, ctypes.POINTER(vp)]
+    L.hydro_batch_destroy.argtypes = [vp]
+    L.hydro_batch_size.argtypes = [vp]
+    L.hydro_batch_device_bytes.argtypes = [vp]
+    L.hydro_batch_device_bytes.restype = sz
+    L.hydro_batch_upload_U.argtypes = [vp, ip, vp, sz]
+    L.hydro_batch_download_U.argtypes = [vp, ip, vp, sz]
+    L.hydro_batch_constrain_U.argtypes = [vp]
+    L.hydro_batch_update.argtypes = [vp, ip, ip]
+    L.hydro_batch_get_time.argtypes = [vp, ip, dp, dp]
+    L.hydro_batch_set_active.argtypes = [vp, ip, ip]
+    L.hydro_batch_sync.argtypes = [vp]
+    L.hydro_batch_time_updates.argtypes = [vp, ip, ip, dp]
+    L.hydro_batch_set_launch.argtypes = [vp, ip, ip]
+    L.hydro_batch_get_launch.argtypes = [vp, ipp, ipp]
+    L.hydro_batch_autotune.argtypes = [vp, ip, ipp, ipp, dp]
     L.hydro_get_launch.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     _lib = L
     return L

@@ -210,6 +210,44 @@ int hydro_get_launch(const hydro_solver *h, int *rows_per_wave, int *xcd_remap);
 /* Bytes of the device state held by the handle. */
 size_t hydro_device_bytes(const hydro_solver *h);
 
+/* ---------------------------------------------------------------- batched grids */
+
+/* Many independent grids advanced together: the reference runs several solvers
+ * from one app loop, each with its own update (hydro/app.lua:1573-1737); here
+ * every stage of every grid of the batch is one persistent launch (one wave per
+ * 60-column strip, strips of all grids dealt to the waves in turn), each grid
+ * keeps its own dt (cfl * its own min, solverbase.lua:3004-3023) and time, and
+ * three steps (one turn of the dt-slot rotation) can be replayed from a
+ * captured hipGraph. All grids share eqn, dim, flux, limiters, plm, integrator,
+ * precision, math and equation parameters; sizes, extents, boundaries and cfl
+ * may differ. Results of a grid do not depend on the other grids. */
+typedef struct hydro_batch hydro_batch;
+
+int hydro_batch_create(const hydro_cfg *cfgs, int n, hydro_batch **out);
+int hydro_batch_destroy(hydro_batch *b);
+int hydro_batch_size(const hydro_batch *b);
+size_t hydro_batch_device_bytes(const hydro_batch *b);
+/* per-grid UBufObj:fromCPU / :toCPU, same layout as hydro_upload_U */
+int hydro_batch_upload_U(hydro_batch *b, int i, const void *U, size_t bytes);
+int hydro_batch_download_U(hydro_batch *b, int i, void *U, size_t bytes);
+int hydro_batch_constrain_U(hydro_batch *b);
+/* nsteps updates of every active grid; use_graph replays captured steps */
+int hydro_batch_update(hydro_batch *b, int nsteps, int use_graph);
+int hydro_batch_get_time(hydro_batch *b, int i, double *t, double *dt);
+/* inactive grids are skipped by the launches (their state and time stay) */
+int hydro_batch_set_active(hydro_batch *b, int i, int active);
+int hydro_batch_sync(hydro_batch *b);
+/* device time of nsteps updates (HIP events on the batch stream) */
+int hydro_batch_time_updates(hydro_batch *b, int nsteps, int use_graph, double *ms);
+/* Launch shape (no counterpart in the reference): rows each wave marches in y
+ * and waves per block (1, 2, 4); 0 keeps the current value. */
+int hydro_batch_set_launch(hydro_batch *b, int rows_per_wave, int waves_per_block);
+int hydro_batch_get_launch(const hydro_batch *b, int *rows_per_wave, int *waves_per_block);
+/* Autotuner (no counterpart in the reference): times trial_steps updates for
+ * each rows-per-wave / waves-per-block candidate (the state advances) and keeps
+ * the fastest; reports the choice and its ms per step. */
+int hydro_batch_autotune(hydro_batch *b, int trial_steps, int *rows_out, int *waves_out, double *ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

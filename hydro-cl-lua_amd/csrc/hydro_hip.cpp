@@ -20,562 +20,25 @@
 #include <string>
 #include <vector>
 
-#include "../../include/hydro_hip.h"
-#include "fv_common.h"
-#include "fv_launch.h"
-#include "fv_aux.h"
+#include "fv_host.h"
 
 using namespace hydro;
 
 // ------------------------------------------------------------------ errors
-static thread_local std::string g_err;
-
-static int fail(const char *fmt, ...) {
-	char buf[1024];
-	va_list ap;
-	va_start(ap, fmt);
-	vsnprintf(buf, sizeof buf, fmt, ap);
-	va_end(ap);
-	g_err = buf;
-	return -1;
+std::string &hydro::hydro_err_buf() {
+	static thread_local std::string e;
+	return e;
 }
 
-#define HIPCHECK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
-	return fail("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); } while (0)
-
-static void ignore(hipError_t) {}
-
-extern "C" const char *hydro_last_error(void) { return g_err.c_str(); }
-
-extern "C" int hydro_eqn_num_states(int eqn) { return eqn == HYDRO_EQN_EULER ? 6 : eqn == HYDRO_EQN_GLM_MAXWELL ? 12 : -1; }
-extern "C" int hydro_eqn_num_int_states(int eqn) { return eqn == HYDRO_EQN_EULER ? 5 : eqn == HYDRO_EQN_GLM_MAXWELL ? 8 : -1; }
+extern "C" const char *hydro_last_error(void) { return hydro_err_buf().c_str(); }
+extern "C" int hydro_eqn_num_states(int eqn) { return num_states(eqn); }
+extern "C" int hydro_eqn_num_int_states(int eqn) { return num_int_states(eqn); }
+extern "C" void hydro_cfg_default(hydro_cfg *c) { cfg_default(c); }
 
 extern "C" int hydro_device_count(void) {
 	int n = 0;
 	if (hipGetDeviceCount(&n) != hipSuccess) return 0;
 	return n;
-}
-
-extern "C" void hydro_cfg_default(hydro_cfg *c) {
-	std::memset(c, 0, sizeof *c);
-	c->dim = 1;
-	for (int i = 0; i < 3; ++i) {
-		c->size[i] = 1;
-		c->mins[i] = -1;
-		c->maxs[i] = 1;
-	}
-	c->eqn = HYDRO_EQN_EULER;
-	c->flux = HYDRO_FLUX_ROE;
-	c->flux_limiter = HYDRO_LIMITER_DONOR_CELL;
-	c->plm = HYDRO_PLM_NONE;
-	c->slope_limiter = HYDRO_LIMITER_DONOR_CELL;
-	c->integrator = HYDRO_INT_FORWARD_EULER;
-	c->cfl = .5;
-	for (int i = 0; i < 6; ++i) c->boundary[i] = HYDRO_BC_FREEFLOW;
-	c->precision = HYDRO_REAL_F64;
-	c->gamma = 7. / 5.;
-	c->rho_min = 1e-7;
-	c->P_min = 1e-7;
-	c->div_phi_wavespeed = 1;
-	c->div_psi_wavespeed = 1;
-}
-
-// --------------------------------------------------------- Shu-Osher tables
-// hydro/int/all.lua:10-160 (rows = stages, columns = k); forward Euler is a
-// one-stage table here (fe.lua:33-49 computes U + dt L(U) as well).
-struct RKTable {
-	int order;
-	double a[4][4], b[4][4];
-};
-
-static bool rk_table(int integ, RKTable &t) {
-	std::memset(&t, 0, sizeof t);
-	switch (integ) {
-	case HYDRO_INT_FORWARD_EULER: t.order = 1; t.a[0][0] = 1; t.b[0][0] = 1; return true;
-	case HYDRO_INT_RK2: t.order = 2;
-		t.a[0][0] = 1; t.a[1][0] = 1; t.b[0][0] = .5; t.b[1][1] = 1; return true;
-	case HYDRO_INT_RK2_HEUN: t.order = 2;
-		t.a[0][0] = 1; t.a[1][0] = 1; t.b[0][0] = 1; t.b[1][0] = .5; t.b[1][1] = .5; return true;
-	case HYDRO_INT_RK2_RALSTON: t.order = 2;
-		t.a[0][0] = 1; t.a[1][0] = 1; t.b[0][0] = 2. / 3.; t.b[1][0] = 1. / 4.; t.b[1][1] = 3. / 4.; return true;
-	case HYDRO_INT_RK3: t.order = 3;
-		t.a[0][0] = t.a[1][0] = t.a[2][0] = 1;
-		t.b[0][0] = .5; t.b[1][0] = -1; t.b[1][1] = 2;
-		t.b[2][0] = 1. / 6.; t.b[2][1] = 2. / 6.; t.b[2][2] = 1. / 6.; return true;
-	case HYDRO_INT_RK4: t.order = 4;
-		for (int i = 0; i < 4; ++i) t.a[i][0] = 1;
-		t.b[0][0] = .5; t.b[1][1] = .5; t.b[2][2] = 1;
-		t.b[3][0] = 1. / 6.; t.b[3][1] = 2. / 6.; t.b[3][2] = 2. / 6.; t.b[3][3] = 1. / 6.; return true;
-	case HYDRO_INT_RK4_3_8THS: t.order = 4;
-		for (int i = 0; i < 4; ++i) t.a[i][0] = 1;
-		t.b[0][0] = 1. / 3.; t.b[1][0] = -1. / 3.;
-		t.b[2][0] = 1; t.b[2][1] = -1; t.b[2][2] = 1;
-		t.b[3][0] = 1. / 8.; t.b[3][1] = 3. / 8.; t.b[3][2] = 3. / 8.; t.b[3][3] = 1. / 8.; return true;
-	case HYDRO_INT_RK2_TVD: t.order = 2;
-		t.a[0][0] = 1; t.a[1][0] = .5; t.a[1][1] = .5; t.b[0][0] = 1; t.b[1][1] = .5; return true;
-	case HYDRO_INT_RK2_NON_TVD: t.order = 2;
-		t.a[0][0] = 1; t.a[1][0] = 1; t.b[0][0] = -20; t.b[1][0] = 41. / 40.; t.b[1][1] = -1. / 40.; return true;
-	case HYDRO_INT_RK3_TVD: t.order = 3;
-		t.a[0][0] = 1; t.a[1][0] = 3. / 4.; t.a[1][1] = 1. / 4.; t.a[2][0] = 1. / 3.; t.a[2][2] = 2. / 3.;
-		t.b[0][0] = 1; t.b[1][1] = 1. / 4.; t.b[2][2] = 2. / 3.; return true;
-	case HYDRO_INT_RK4_TVD: t.order = 4;
-		t.a[0][0] = 1;
-		t.a[1][0] = 649. / 1600.; t.a[1][1] = 951. / 1600.;
-		t.a[2][0] = 53989. / 2500000.; t.a[2][1] = 4806213. / 20000000.; t.a[2][2] = 23619. / 32000.;
-		t.a[3][0] = 1. / 5.; t.a[3][1] = 6127. / 30000.; t.a[3][2] = 7873. / 30000.; t.a[3][3] = 1. / 3.;
-		t.b[0][0] = .5;
-		t.b[1][0] = -10890423. / 25193600.; t.b[1][1] = 5000. / 7873;
-		t.b[2][0] = -102261. / 5000000.; t.b[2][1] = -5121. / 20000.; t.b[2][2] = 7873. / 10000.;
-		t.b[3][0] = 1. / 10.; t.b[3][1] = 1. / 6.; t.b[3][3] = 1. / 6.; return true;
-	case HYDRO_INT_RK4_NON_TVD: t.order = 4;
-		for (int i = 0; i < 3; ++i) t.a[i][0] = 1;
-		t.a[3][0] = -1. / 3.; t.a[3][1] = 1. / 3.; t.a[3][2] = 2. / 3.; t.a[3][3] = 1. / 3.;
-		t.b[0][0] = .5; t.b[1][1] = .5; t.b[2][2] = 1; t.b[3][3] = 1. / 6.; return true;
-	}
-	return false;
-}
-
-static bool needs_deriv(const RKTable &t, int i) {
-	for (int m = i + 1; m < t.order; ++m)
-		if (t.b[m][i] != 0) return true;
-	return false;
-}
-
-// ------------------------------------------------------------------ Grid
-struct Grid {
-	hydro_cfg cfg;
-	int dim, Q, NSTAT, S;
-	int n[3];
-	long ncell;
-	long ps;
-	size_t elem;
-	RKTable rk;
-	void *U[4];
-	void *D[4];
-	void *ping;
-	void *stat;
-	DevScalars *dev;
-	int slot;           // dt slot of the next step
-	int dtValid;        // slot holds the min for the current state
-	int rowsPerWave;
-	int nvel;           // Euler: velocity components that are not identically zero
-	std::vector<double> ePot;   // Euler ePot of the last upload (dynamics-neutral)
-};
-
-static size_t elem_size(const hydro_cfg &c) { return c.precision == HYDRO_REAL_F32 ? 4 : 8; }
-
-static int validate(const hydro_cfg *c) {
-	if (!c) return fail("cfg is NULL");
-	if (c->dim < 1 || c->dim > 2) return fail("dim %d not supported (1 or 2)", c->dim);
-	for (int i = 0; i < c->dim; ++i)
-		if (c->size[i] < 1) return fail("size[%d] = %d must be >= 1", i, c->size[i]);
-	if (c->eqn != HYDRO_EQN_EULER && c->eqn != HYDRO_EQN_GLM_MAXWELL) return fail("eqn %d not supported", c->eqn);
-	if (c->flux != HYDRO_FLUX_ROE) return fail("flux %d not supported", c->flux);
-	if (c->flux_limiter < 0 || c->flux_limiter >= HYDRO_NUM_LIMITERS) return fail("flux_limiter %d out of range", c->flux_limiter);
-	if (c->slope_limiter < 0 || c->slope_limiter >= HYDRO_NUM_LIMITERS) return fail("slope_limiter %d out of range", c->slope_limiter);
-	if (c->plm != HYDRO_PLM_NONE && c->plm != HYDRO_PLM_CONS) return fail("plm %d not supported", c->plm);
-	// gridsolver.lua:119
-	if (c->plm && c->flux_limiter != HYDRO_LIMITER_DONOR_CELL)
-		return fail("are you sure you want to use flux and slope limiters at the same time?");
-	RKTable t;
-	if (!rk_table(c->integrator, t)) return fail("integrator %d not supported", c->integrator);
-	for (int f = 0; f < 2 * c->dim; ++f)
-		if (c->boundary[f] < 0 || c->boundary[f] > HYDRO_BC_FIXED) return fail("boundary[%d] = %d invalid", f, c->boundary[f]);
-	if (c->precision != HYDRO_REAL_F64 && c->precision != HYDRO_REAL_F32) return fail("precision %d invalid", c->precision);
-	if (!(c->cfl > 0) && !c->use_fixed_dt) return fail("cfl must be > 0");
-	return 0;
-}
-
-static int default_rows(int dim, int nx, int ny, long targetWaves) {
-	// enough waves to fill the device without letting the 2-3 row warm-up of
-	// the y pipeline dominate
-	if (dim < 2) return 1;
-	long stripsX = (nx + STRIP_OUT - 1) / STRIP_OUT;
-	long rows = (stripsX * (long)ny + targetWaves - 1) / targetWaves;
-	return (int)std::max(4L, std::min(64L, rows));
-}
-
-static void grid_init(Grid &g, const hydro_cfg &cfg) {
-	g.cfg = cfg;
-	g.dim = cfg.dim;
-	g.Q = hydro_eqn_num_int_states(cfg.eqn);
-	g.S = hydro_eqn_num_states(cfg.eqn);
-	g.NSTAT = cfg.eqn == HYDRO_EQN_GLM_MAXWELL ? 4 : 0;
-	g.ncell = 1;
-	for (int i = 0; i < 3; ++i) {
-		g.n[i] = i < g.dim ? cfg.size[i] : 1;
-		g.ncell *= g.n[i];
-	}
-	g.ps = g.ncell;
-	g.elem = elem_size(cfg);
-	rk_table(cfg.integrator, g.rk);
-	std::memset(g.U, 0, sizeof g.U);
-	std::memset(g.D, 0, sizeof g.D);
-	g.ping = g.stat = nullptr;
-	g.dev = nullptr;
-	g.slot = 0;
-	g.dtValid = 0;
-	g.nvel = 3;
-	g.rowsPerWave = default_rows(g.dim, g.n[0], g.n[1], 4096);
-}
-
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-// device bytes of a grid, in the order grid_bind lays them out
-static size_t grid_bytes(const Grid &g) {
-	size_t plane = (size_t)g.ps * g.elem, state = align256(plane * g.Q);
-	size_t b = align256(sizeof(DevScalars));
-	int o = g.rk.order;
-	for (int i = 0; i < o; ++i) {
-		b += state;
-		if (needs_deriv(g.rk, i)) b += state;
-	}
-	if (o == 1) b += state;
-	if (g.NSTAT) b += align256(plane * g.NSTAT);
-	return b;
-}
-
-static void grid_bind(Grid &g, char *base) {
-	size_t plane = (size_t)g.ps * g.elem, state = align256(plane * g.Q);
-	char *p = base;
-	g.dev = (DevScalars *)p;
-	p += align256(sizeof(DevScalars));
-	int o = g.rk.order;
-	for (int i = 0; i < o; ++i) {
-		g.U[i] = p;
-		p += state;
-		if (needs_deriv(g.rk, i)) {
-			g.D[i] = p;
-			p += state;
-		}
-	}
-	if (o == 1) {
-		g.ping = p;
-		p += state;
-	}
-	if (g.NSTAT) g.stat = p;
-}
-
-// zero every state buffer, +inf every dt slot (asynchronous on s)
-static int grid_clear(Grid &g, char *base, hipStream_t s) {
-	HIPCHECK(hipMemsetAsync(base, 0, grid_bytes(g), s));
-	DevScalars z;
-	std::memset(&z, 0, sizeof z);
-	for (int i = 0; i < 3; ++i)
-		for (int j = 0; j < DT_LANES; ++j) z.dtmin[i][j] = INFINITY;
-	HIPCHECK(hipMemcpyAsync(g.dev, &z, sizeof z, hipMemcpyHostToDevice, s));
-	return 0;
-}
-
-template <typename R>
-static void grid_desc(const Grid &g, GridDesc<R> &d) {
-	std::memset(&d, 0, sizeof d);
-	for (int i = 0; i < 4; ++i) {
-		d.U[i] = (R *)g.U[i];
-		d.D[i] = (R *)g.D[i];
-	}
-	d.ping = (R *)g.ping;
-	d.stat = (const R *)g.stat;
-	for (int i = 0; i < 3; ++i) d.n[i] = g.n[i];
-	d.ps = g.ps;
-	d.step1 = g.n[0];
-	const hydro_cfg &c = g.cfg;
-	for (int f = 0; f < 6; ++f) {
-		d.bc[f] = c.boundary[f];
-		for (int j = 0; j < g.Q + g.NSTAT; ++j) d.fixedv[f][j] = R(c.boundary_fixed[f][j]);
-	}
-	// fvsolver.cl:37-102 in the stored precision: volume = 1 * dx0 * dx1...,
-	// areaL = 1 * prod(dx_other), areaL *= 1 / volume
-	R dxr[3];
-	for (int i = 0; i < 3; ++i) {
-		dxr[i] = R((c.maxs[i] - c.mins[i]) / (double)g.n[i]);
-		d.dx[i] = dxr[i];
-		d.invdx[i] = R(1) / dxr[i];
-		d.dxD[i] = (c.maxs[i] - c.mins[i]) / (double)g.n[i];
-	}
-	R volume = R(1);
-	for (int i = 0; i < g.dim; ++i) volume = volume * dxr[i];
-	d.derivOn = volume > R(1e-7);
-	R invVolume = R(1) / volume;
-	for (int side = 0; side < 3; ++side) {
-		R area = R(1);
-		for (int i = 0; i < g.dim; ++i)
-			if (i != side) area = area * dxr[i];
-		R aL = area;
-		if (aL <= R(1e-7)) aL = R(0);
-		d.areaOverVol[side] = aL * invVolume;
-		d.fluxOn[side] = area > R(1e-7);
-	}
-	d.cfl = c.cfl;
-	d.rowsPerWave = g.rowsPerWave;
-	d.stripsX = (g.n[0] + STRIP_OUT - 1) / STRIP_OUT;
-	d.stripsY = g.dim >= 2 ? (g.n[1] + g.rowsPerWave - 1) / g.rowsPerWave : 1;
-	d.nItems = (long)d.stripsX * d.stripsY;
-	d.active = 1;
-	d.itemBase = 0;
-	d.dev = g.dev;
-	d.stepsLeft = 1 << 30;
-}
-
-// Uniform arguments of stage s (0-based) of the table.
-template <typename R>
-static void stage_args(const Grid &g, int s, StageArgs<R> &a) {
-	std::memset(&a, 0, sizeof a);
-	const RKTable &t = g.rk;
-	int o = t.order;
-	a.src = s;
-	bool last = s == o - 1;
-	a.out = !last ? s + 1 : (o == 1 ? -1 : 0);
-	a.nAlpha = 0;
-	for (int k = 0; k <= s; ++k) {
-		if (t.a[s][k] != 0) {
-			a.accU[a.nAlpha] = k;
-			a.alpha[a.nAlpha] = t.a[s][k];
-			a.nAlpha++;
-		}
-	}
-	a.nStoredD = 0;
-	for (int k = 0; k < s; ++k) {
-		if (t.b[s][k] != 0) {
-			a.accD[a.nStoredD] = k;
-			a.beta[a.nStoredD] = t.b[s][k];
-			a.nStoredD++;
-		}
-	}
-	a.storeD = needs_deriv(t, s) ? s : -1;
-	a.betaNew = t.b[s][s];
-	const hydro_cfg &c = g.cfg;
-	a.gamma = R(c.gamma);
-	a.rhoMin = R(c.rho_min);
-	a.PMin = R(c.P_min);
-	a.divPhi = R(c.div_phi_wavespeed);
-	a.divPsi = R(c.div_psi_wavespeed);
-	a.gammaD = c.gamma;
-	a.rhoMinD = c.rho_min;
-	a.PMinD = c.P_min;
-	a.divPhiD = c.div_phi_wavespeed;
-	a.divPsiD = c.div_psi_wavespeed;
-	a.limiter = c.plm ? c.slope_limiter : c.flux_limiter;
-	a.resetSlot = -1;
-	a.reduceSlot = -1;
-	a.dtSlot = -1;
-	a.advanceTime = 0;
-	a.xcdRemap = 1;
-}
-
-static StageKey stage_key(const Grid &g) {
-	StageKey k;
-	k.eqn = g.cfg.eqn;
-	k.dim = g.dim;
-	k.plm = g.cfg.plm != HYDRO_PLM_NONE;
-	k.lim = !k.plm && g.cfg.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
-	k.limiter = k.plm ? g.cfg.slope_limiter : g.cfg.flux_limiter;
-	k.fast = g.cfg.fast_math != 0;
-	k.nvel = g.nvel;
-	return k;
-}
-
-// ------------------------------------------------------- layout conversion
-// Ghost coordinate -> interior source, following the reference's x, y, z
-// boundary passes (gridsolver.lua:1272-1314).
-struct HostMap {
-	int idx, mirror, fixedFace;
-};
-static HostMap host_map(int c, int n, int kmin, int kmax, int faceMin) {
-	HostMap m{c, 0, -1};
-	if (c < 0) {
-		switch (kmin) {
-		case HYDRO_BC_PERIODIC: m.idx = ((c % n) + n) % n; break;
-		case HYDRO_BC_MIRROR: m.idx = -1 - c; m.mirror = 1; break;
-		case HYDRO_BC_FREEFLOW: m.idx = 0; break;
-		default: m.idx = 0; m.fixedFace = faceMin; break;
-		}
-	} else if (c >= n) {
-		switch (kmax) {
-		case HYDRO_BC_PERIODIC: m.idx = c % n; break;
-		case HYDRO_BC_MIRROR: m.idx = 2 * n - 1 - c; m.mirror = 1; break;
-		case HYDRO_BC_FREEFLOW: m.idx = n - 1; break;
-		default: m.idx = n - 1; m.fixedFace = faceMin + 1; break;
-		}
-	}
-	m.idx = std::max(0, std::min(n - 1, m.idx));
-	return m;
-}
-
-static bool reflects(int eqn, int field, int axis) {
-	if (eqn == HYDRO_EQN_EULER) return field == 1 + axis;
-	return field == axis || field == 3 + axis;
-}
-
-static size_t ghosted_cells(const Grid &g) {
-	size_t c = 1;
-	for (int i = 0; i < 3; ++i) c *= (size_t)(i < g.dim ? g.n[i] + 2 * NG : 1);
-	return c;
-}
-
-template <typename R>
-static void pack_interior(Grid &g, const R *aos, std::vector<R> &planes, std::vector<R> &stat) {
-	int gs[3], o[3];
-	for (int i = 0; i < 3; ++i) {
-		gs[i] = i < g.dim ? g.n[i] + 2 * NG : 1;
-		o[i] = i < g.dim ? NG : 0;
-	}
-	planes.assign((size_t)g.Q * g.ps, R(0));
-	stat.assign((size_t)g.NSTAT * g.ps, R(0));
-	bool anyE = false;
-	if (g.cfg.eqn == HYDRO_EQN_EULER) g.ePot.assign(g.ncell, 0.);
-	for (int k = 0; k < g.n[2]; ++k)
-		for (int j = 0; j < g.n[1]; ++j)
-			for (int i = 0; i < g.n[0]; ++i) {
-				size_t gi = (size_t)(i + o[0]) + (size_t)gs[0] * ((j + o[1]) + (size_t)gs[1] * (k + o[2]));
-				size_t c = (size_t)i + (size_t)g.n[0] * (j + (size_t)g.n[1] * k);
-				const R *u = aos + gi * g.S;
-				for (int f = 0; f < g.Q; ++f) planes[(size_t)f * g.ps + c] = u[f];
-				for (int f = 0; f < g.NSTAT; ++f) stat[(size_t)f * g.ps + c] = u[g.Q + f];
-				if (g.cfg.eqn == HYDRO_EQN_EULER) {
-					g.ePot[c] = (double)u[5];
-					anyE = anyE || u[5] != R(0);
-				}
-			}
-	if (!anyE) g.ePot.clear();
-	// Euler: momentum components beyond dim that are zero everywhere stay zero
-	// (their fluxes and wave amplitudes are exact zeros), so the kernels skip them
-	g.nvel = 3;
-	if (g.cfg.eqn == HYDRO_EQN_EULER && g.dim < 3) {
-		bool zero = true;
-		for (int f = 1 + g.dim; f <= 3 && zero; ++f)
-			for (long c = 0; c < g.ps; ++c)
-				if (planes[(size_t)f * g.ps + c] != R(0)) { zero = false; break; }
-		if (zero) g.nvel = g.dim;
-	}
-}
-
-template <typename R>
-static void unpack_with_ghosts(const Grid &g, const std::vector<R> &planes, const std::vector<R> &stat, R *aos) {
-	int gs[3];
-	for (int i = 0; i < 3; ++i) gs[i] = i < g.dim ? g.n[i] + 2 * NG : 1;
-	const hydro_cfg &c = g.cfg;
-	for (int k = 0; k < gs[2]; ++k)
-		for (int j = 0; j < gs[1]; ++j)
-			for (int i = 0; i < gs[0]; ++i) {
-				int cc[3] = {i - (g.dim > 0 ? NG : 0), j - (g.dim > 1 ? NG : 0), k - (g.dim > 2 ? NG : 0)};
-				HostMap m[3];
-				for (int a = 0; a < 3; ++a) {
-					if (a < g.dim) m[a] = host_map(cc[a], g.n[a], c.boundary[2 * a], c.boundary[2 * a + 1], 2 * a);
-					else m[a] = HostMap{0, 0, -1};
-				}
-				size_t src = (size_t)m[0].idx + (size_t)g.n[0] * (m[1].idx + (size_t)g.n[1] * m[2].idx);
-				R *u = aos + ((size_t)i + (size_t)gs[0] * (j + (size_t)gs[1] * k)) * g.S;
-				for (int f = 0; f < g.S; ++f) {
-					R v;
-					if (f < g.Q) v = planes[(size_t)f * g.ps + src];
-					else if (f - g.Q < g.NSTAT) v = stat[(size_t)(f - g.Q) * g.ps + src];
-					else v = (g.cfg.eqn == HYDRO_EQN_EULER && !g.ePot.empty()) ? R(g.ePot[src]) : R(0);
-					for (int a = 0; a < g.dim; ++a) {
-						if (m[a].fixedFace >= 0) v = R(c.boundary_fixed[m[a].fixedFace][f]);
-						else if (m[a].mirror && reflects(c.eqn, f, a)) v = -v;
-					}
-					u[f] = v;
-				}
-			}
-}
-
-template <typename R>
-static int grid_upload_t(Grid &g, const void *U, hipStream_t s) {
-	std::vector<R> planes, stat;
-	pack_interior<R>(g, (const R *)U, planes, stat);
-	HIPCHECK(hipMemcpyAsync(g.U[0], planes.data(), planes.size() * sizeof(R), hipMemcpyHostToDevice, s));
-	if (g.NSTAT) HIPCHECK(hipMemcpyAsync(g.stat, stat.data(), stat.size() * sizeof(R), hipMemcpyHostToDevice, s));
-	HIPCHECK(hipStreamSynchronize(s));
-	g.dtValid = 0;
-	return 0;
-}
-
-template <typename R>
-static int grid_download_t(const Grid &g, void *U, hipStream_t s) {
-	std::vector<R> planes((size_t)g.Q * g.ps), stat((size_t)g.NSTAT * g.ps);
-	HIPCHECK(hipMemcpyAsync(planes.data(), g.U[0], planes.size() * sizeof(R), hipMemcpyDeviceToHost, s));
-	if (g.NSTAT) HIPCHECK(hipMemcpyAsync(stat.data(), g.stat, stat.size() * sizeof(R), hipMemcpyDeviceToHost, s));
-	HIPCHECK(hipStreamSynchronize(s));
-	unpack_with_ghosts<R>(g, planes, stat, (R *)U);
-	return 0;
-}
-
-static int grid_upload(Grid &g, const void *U, size_t bytes, hipStream_t s) {
-	if (!U) return fail("NULL argument");
-	size_t need = ghosted_cells(g) * g.S * g.elem;
-	if (bytes != need) return fail("upload size %zu != %zu (numCells * numStates * sizeof(real))", bytes, need);
-	return g.elem == 8 ? grid_upload_t<double>(g, U, s) : grid_upload_t<float>(g, U, s);
-}
-
-static int grid_download(const Grid &g, void *U, size_t bytes, hipStream_t s) {
-	if (!U) return fail("NULL argument");
-	size_t need = ghosted_cells(g) * g.S * g.elem;
-	if (bytes != need) return fail("download size %zu != %zu (numCells * numStates * sizeof(real))", bytes, need);
-	return g.elem == 8 ? grid_download_t<double>(g, U, s) : grid_download_t<float>(g, U, s);
-}
-
-template <typename R>
-static GridArgs<R> grid_args(const Grid &g, int slot) {
-	GridArgs<R> a;
-	std::memset(&a, 0, sizeof a);
-	a.U = (R *)g.U[0];
-	a.stat = (const R *)g.stat;
-	a.ps = g.ps;
-	a.ncell = g.ncell;
-	a.dim = g.dim;
-	for (int i = 0; i < 3; ++i) a.dx[i] = (g.cfg.maxs[i] - g.cfg.mins[i]) / (double)g.n[i];
-	a.gamma = g.cfg.gamma;
-	a.rhoMin = g.cfg.rho_min;
-	a.PMin = g.cfg.P_min;
-	a.divPhi = g.cfg.div_phi_wavespeed;
-	a.divPsi = g.cfg.div_psi_wavespeed;
-	a.dev = g.dev;
-	a.slot = slot;
-	return a;
-}
-
-static int grid_dt_kernel(Grid &g, int slot, hipStream_t s) {
-	HIPCHECK(launch_reset_slots(g.dev, 1 << slot, s));
-	hipError_t e = g.elem == 8 ? launch_dt<double>(g.cfg.eqn, g.dim, g.cfg.fast_math, grid_args<double>(g, slot), s)
-	                           : launch_dt<float>(g.cfg.eqn, g.dim, g.cfg.fast_math, grid_args<float>(g, slot), s);
-	if (e != hipSuccess) return fail("dt kernel: %s", hipGetErrorString(e));
-	return 0;
-}
-
-static int grid_constrain(Grid &g, hipStream_t s) {
-	hipError_t e = g.elem == 8 ? launch_constrain<double>(g.cfg.eqn, g.cfg.fast_math, grid_args<double>(g, 0), s)
-	                           : launch_constrain<float>(g.cfg.eqn, g.cfg.fast_math, grid_args<float>(g, 0), s);
-	if (e != hipSuccess) return fail("constrainU kernel: %s", hipGetErrorString(e));
-	g.dtValid = 0;
-	return 0;
-}
-
-static int grid_calc_dt(Grid &g, double *dt, hipStream_t s) {
-	if (g.cfg.use_fixed_dt) { *dt = g.cfg.fixed_dt; return 0; }
-	if (grid_dt_kernel(g, g.slot, s)) return -1;
-	HIPCHECK(launch_fold_slot(g.dev, g.slot, s));
-	double m;
-	HIPCHECK(hipMemcpyAsync(&m, &g.dev->dtmin[g.slot][0], sizeof m, hipMemcpyDeviceToHost, s));
-	HIPCHECK(hipStreamSynchronize(s));
-	g.dtValid = 1;
-	*dt = g.cfg.cfl * m;
-	return 0;
-}
-
-static int grid_get_time(const Grid &g, double *t, double *dt, hipStream_t s) {
-	double v[2];
-	HIPCHECK(hipMemcpyAsync(v, &g.dev->t, sizeof v, hipMemcpyDeviceToHost, s));
-	HIPCHECK(hipStreamSynchronize(s));
-	if (t) *t = v[0];
-	if (dt) *dt = v[1];
-	return 0;
-}
-
-static int grid_set_time(Grid &g, double t, hipStream_t s) {
-	HIPCHECK(hipMemcpyAsync(&g.dev->t, &t, sizeof t, hipMemcpyHostToDevice, s));
-	HIPCHECK(hipStreamSynchronize(s));
-	return 0;
 }
 
 // ------------------------------------------------------------------ solver
@@ -893,30 +356,10 @@ static int batch_write_descs(hydro_batch *b) {
 
 template <typename R>
 static int batch_stage(hydro_batch *b, int s, int cur) {
-	const Grid &g0 = b->grids[0];
-	int o = g0.rk.order;
-	int fixed = g0.cfg.use_fixed_dt;
-	bool last = s == o - 1;
-	StageArgs<R> a;
-	stage_args<R>(g0, s, a);
-	if (!fixed) a.dtSlot = cur;
-	else a.hostDt = g0.cfg.fixed_dt;
-	a.resetSlot = (!fixed && s == 0) ? (cur + 2) % 3 : -1;
-	a.reduceSlot = (!fixed && last) ? (cur + 1) % 3 : -1;
-	a.advanceTime = last;
-	a.stepInTick = 0;
-	a.grids = (const GridDesc<R> *)b->descs;
-	a.nGrids = (int)b->grids.size();
-	a.totalItems = batch_items(b);
-	a.xcdRemap = b->xcdRemap;
-	StageKey k = stage_key(g0);
-	// one kernel for every grid: take the widest velocity set
 	int nv = 0;
 	for (auto &g : b->grids) nv = std::max(nv, g.nvel);
-	k.nvel = nv;
-	hipError_t e = launch_stage<R>(k, a, dim3(0), dim3(64 * b->wavesPerBlock), b->stream);
-	if (e != hipSuccess) return fail("batched stage launch: %s", hipGetErrorString(e));
-	return 0;
+	return launch_batched_stage<R>(b->grids[0], nv, s, cur, (const GridDesc<R> *)b->descs, (int)b->grids.size(),
+		batch_items(b), 0, b->wavesPerBlock, b->xcdRemap, b->stream);
 }
 
 // One step of every active grid with the slots rotated to cur.
@@ -939,7 +382,7 @@ extern "C" int hydro_batch_create(const hydro_cfg *cfgs, int n, hydro_batch **ou
 	*out = nullptr;
 	if (!cfgs || n < 1) return fail("need at least one grid");
 	for (int i = 0; i < n; ++i) {
-		if (validate(&cfgs[i])) return fail("grid %d: %s", i, g_err.c_str());
+		if (validate(&cfgs[i])) return fail("grid %d: %s", i, hydro_err_buf().c_str());
 		if (!same_kernel_config(cfgs[0], cfgs[i]))
 			return fail("grid %d: a batch shares eqn, dim, flux, limiters, plm, integrator, precision, math and "
 			            "equation parameters", i);

@@ -22,7 +22,15 @@ EXPORTS = [
     'hydro_batch_upload_U', 'hydro_batch_download_U', 'hydro_batch_constrain_U', 'hydro_batch_update',
     'hydro_batch_get_time', 'hydro_batch_set_active', 'hydro_batch_sync', 'hydro_batch_time_updates',
     'hydro_batch_set_launch', 'hydro_batch_get_launch', 'hydro_batch_autotune',
+    'hydro_server_create', 'hydro_server_destroy', 'hydro_server_submit', 'hydro_server_tick',
+    'hydro_server_poll', 'hydro_server_result', 'hydro_server_info_get', 'hydro_server_set_launch',
 ]
+
+
+class HydroServerInfo(ctypes.Structure):
+    _fields_ = [('queued', ctypes.c_int), ('resident', ctypes.c_int), ('finished', ctypes.c_int),
+                ('pools', ctypes.c_int), ('bytes_in_use', ctypes.c_size_t), ('budget', ctypes.c_size_t),
+                ('ticks', ctypes.c_longlong)]
 
 
 class HydroTiming(ctypes.Structure):
@@ -90,6 +98,15 @@ def lib():
     L.hydro_batch_set_launch.argtypes = [vp, ip, ip]
     L.hydro_batch_get_launch.argtypes = [vp, ipp, ipp]
     L.hydro_batch_autotune.argtypes = [vp, ip, ipp, ipp, dp]
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    L.hydro_server_create.argtypes = [sz, ip, ctypes.POINTER(vp)]
+    L.hydro_server_destroy.argtypes = [vp]
+    L.hydro_server_submit.argtypes = [vp, cfgp, vp, sz, ip, llp]
+    L.hydro_server_tick.argtypes = [vp, ipp]
+    L.hydro_server_poll.argtypes = [vp, llp, ip, ipp]
+    L.hydro_server_result.argtypes = [vp, ctypes.c_longlong, vp, sz, dp, ipp]
+    L.hydro_server_info_get.argtypes = [vp, ctypes.POINTER(HydroServerInfo)]
+    L.hydro_server_set_launch.argtypes = [vp, ip, ip]
     L.hydro_get_launch.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip)]
     _lib = L
     return L

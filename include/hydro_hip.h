@@ -248,6 +248,46 @@ int hydro_batch_get_launch(const hydro_batch *b, int *rows_per_wave, int *waves_
  * the fastest; reports the choice and its ms per step. */
 int hydro_batch_autotune(hydro_batch *b, int trial_steps, int *rows_out, int *waves_out, double *ms_per_step);
 
+/* ---------------------------------------------------------------- solve server */
+
+/* Continuous batching of solve requests (no counterpart in the reference; its
+ * app loop steps a fixed set of solvers, hydro/app.lua:1573-1737). Requests
+ * (configuration + initial state + step count) queue on submit; each tick
+ * admits queued requests in arrival order while their device state fits the
+ * HBM budget (ones that do not fit wait, smaller ones may pass them), advances
+ * every resident request by up to steps_per_tick steps — requests of one kernel
+ * configuration share persistent batched launches, configurations run on their
+ * own streams and overlap — and retires finished ones: the final state is
+ * copied to the host and the memory freed. A result equals the one of the same
+ * request solved alone. The server runs the Runge-Kutta integrators. */
+typedef struct hydro_server hydro_server;
+
+typedef struct hydro_server_info {
+	int queued;              /* waiting for memory */
+	int resident;            /* on the device */
+	int finished;            /* done, result not yet fetched */
+	int pools;               /* kernel configurations seen */
+	size_t bytes_in_use;
+	size_t budget;
+	long long ticks;
+} hydro_server_info;
+
+int hydro_server_create(size_t hbm_budget_bytes, int steps_per_tick, hydro_server **out);
+int hydro_server_destroy(hydro_server *s);
+/* U: initial ghosted cons_t buffer as for hydro_upload_U. Fails when the
+ * request alone needs more device memory than the budget. */
+int hydro_server_submit(hydro_server *s, const hydro_cfg *cfg, const void *U, size_t bytes, int nsteps,
+	long long *id);
+int hydro_server_tick(hydro_server *s, int *n_completed);
+/* ids of finished requests whose results were not fetched yet (up to max) */
+int hydro_server_poll(hydro_server *s, long long *ids, int max, int *n);
+/* final state (ghosted, as hydro_download_U), time, and status: 0 ok,
+ * 1 non-finite values in the state. Releases the request. */
+int hydro_server_result(hydro_server *s, long long id, void *U, size_t bytes, double *t, int *status);
+int hydro_server_info_get(hydro_server *s, hydro_server_info *info);
+/* launch shape of every pool (no counterpart in the reference) */
+int hydro_server_set_launch(hydro_server *s, int rows_per_wave, int waves_per_block);
+
 #ifdef __cplusplus
 }
 #endif

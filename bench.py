@@ -1,12 +1,26 @@
-"""Benchmark: 2D Euler Kelvin-Helmholtz, 1024x1024 periodic, Roe + plm cons
-(minmod), RK2-TVD, CFL 0.4, FP64 — the plain single-grid form of the
-finite-volume hot path on one MI355X.
+"""Benchmark of the finite-volume hot path on one MI355X.
 
-A step is one full RK2-TVD update of the 1024^2 grid (two fused stage
-launches; dt stays on the device). The state is resident in HBM before the
-timed region. Prints one JSON line with Gcell-updates/s, the roofline of the
-stage kernel (HIP events around each launch on the solver stream) and the CPU
-oracle timed on the host cores on a bounded sample of the same workload.
+Headline (`value`): 2D Euler Kelvin-Helmholtz, 1024x1024 periodic, Roe +
+plm cons (minmod), RK2-TVD, CFL 0.4, FP64, in Gcell-updates/s. A step is one
+full RK2-TVD update of the grid (two fused stage launches; dt stays on the
+device). The state is resident in HBM before the timed region, which holds
+exactly --steps steps between two device synchronizations.
+
+Beside it on the same JSON line:
+  roofline      the stage kernel: algorithmic bytes per launch / average launch
+                duration (HIP events around each launch on the solver stream)
+  fp32          the same KH problem in the FP32 bulk form, and its relative L1
+                difference per conserved variable from the FP64 result
+  batched       512 independent 2D Euler four-quadrant grids (sides log-uniform
+                in [64, 512]), FP32, RK2-TVD, 300 steps: one persistent launch
+                per stage over all grids, hipGraph replay, autotuned launch
+                shape; solves/s against the same grids advanced one after the
+                other by the plain solver, and the FP32 error against FP64
+  serving       Poisson stream of mixed 1D Euler / 2D Euler / 2D GLM-Maxwell
+                solves at 80% of the measured closed-loop capacity: solves/s,
+                p50 / p99 latency, against one-at-a-time plain solves
+  cpu_baseline  the CPU oracle (FP64 reference kernel sequence, OpenMP) on a
+                bounded sample of the headline workload
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20]
 """
@@ -15,6 +29,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, 'hydro-cl-lua_amd'))
@@ -29,6 +45,21 @@ def algorithmic_bytes_per_cell_update(w, stages_rw):
     stage 1 reads U^n and writes U^1, stage 2 reads U^1 and U^n and writes
     U^{n+1}: 5 q w"""
     return stages_rw * NQ * w
+
+
+def rel_l1(got, ref):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    ax = tuple(range(ref.ndim - 1))
+    den = np.abs(ref).sum(axis=ax)
+    return np.abs(got - ref).sum(axis=ax) / np.where(den == 0, 1, den)
+
+
+def interior(U, dim):
+    sl = [slice(None)] * 4
+    for i in range(dim):
+        sl[2 - i] = slice(2, -2)
+    return U[tuple(sl)]
 
 
 def cpu_baseline(seconds=10.0):
@@ -69,6 +100,131 @@ def cpu_baseline(seconds=10.0):
                       f'oracle = reference kernel sequence, AoS, unfused, OpenMP; cpu: {cpu}'}
 
 
+def kh_solver(n, real, fast, rows=0):
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as init_euler
+    s = HipFVSolver(dim=2, gridSize=(n, n), mins=(-.5, -.5), maxs=(.5, .5), usePLM='plm cons',
+                    slopeLimiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4,
+                    boundary='periodic', real=real, fastMath=fast)
+    if rows:
+        s.setLaunch(rows, 0)
+    U0 = init_euler.to_ghosted(s.cfg, init_euler.kelvin_helmholtz(s.cfg, seed=0))
+    s.resetState(U0.astype(s.dtype))
+    return s
+
+
+def timed_steps(s, warmup, steps):
+    s.update(max(warmup, 1))
+    s.sync()
+    t0 = time.perf_counter()
+    s.update(steps)
+    s.sync()
+    return time.perf_counter() - t0
+
+
+def batched_leg(count, steps, seed=0, accuracy_grids=8):
+    """solves/s of the batched form vs the plain solver one grid at a time"""
+    from hydro import workloads as wl
+    from hydro.solver.batch import HipBatch
+    from hydro.solver.hipfvsolver import HipFVSolver
+    ens = wl.ensemble_2d(count, seed=seed)
+    cells = sum(kw['size'][0] * kw['size'][1] for kw, _ in ens)
+    b = HipBatch([kw for kw, _ in ens])
+
+    def load():
+        for i, (_, U0) in enumerate(ens):
+            b.upload(i, U0.astype(np.float32))
+        b.constrainU()
+
+    load()
+    t0 = time.perf_counter()
+    rows, wpb, _ = b.autotune(3)
+    tune_s = time.perf_counter() - t0
+    b.setLaunch(rows, wpb)
+    load()
+    b.update(3)                 # capture and prime the graph outside the timed region
+    load()
+    b.sync()
+    t0 = time.perf_counter()
+    b.update(steps, graph=True)
+    b.sync()
+    wall = time.perf_counter() - t0
+    out = [b.download(i) for i in range(count)]
+
+    # the same grids advanced one after the other by the plain solver (state resident)
+    plains = []
+    for kw, U0 in ens:
+        s = HipFVSolver(dim=2, gridSize=kw['size'], mins=kw['mins'], maxs=kw['maxs'], usePLM=kw['plm'],
+                        slopeLimiter=kw['slope_limiter'], integrator=kw['integrator'], cfl=kw['cfl'],
+                        boundary=kw['boundary'], real='float', fastMath=True)
+        s.resetState(U0.astype(np.float32))
+        plains.append(s)
+    for s in plains:
+        s.sync()
+    t0 = time.perf_counter()
+    for s in plains:           # one solve after the other: each finishes before the next starts
+        s.update(steps)
+        s.sync()
+    seq_wall = time.perf_counter() - t0
+    same = all(np.array_equal(out[i], plains[i].UBufObj.toCPU()) for i in range(0, count, max(1, count // 16)))
+    del plains
+
+    # FP32 error against FP64 (IEEE) on a sample of the grids
+    rng = np.random.default_rng(seed + 99)
+    pick = sorted(rng.choice(count, size=min(accuracy_grids, count), replace=False).tolist())
+    worst = 0.
+    for i in pick:
+        kw, U0 = ens[i]
+        s = HipFVSolver(dim=2, gridSize=kw['size'], mins=kw['mins'], maxs=kw['maxs'], usePLM=kw['plm'],
+                        slopeLimiter=kw['slope_limiter'], integrator=kw['integrator'], cfl=kw['cfl'],
+                        boundary=kw['boundary'], real='double', fastMath=False)
+        s.resetState(U0.astype(np.float32).astype(np.float64))
+        s.update(steps)
+        e = rel_l1(interior(out[i], 2)[..., :NQ], interior(s.UBufObj.toCPU(), 2)[..., :NQ])
+        worst = max(worst, float(e.max()))
+        del s
+    return {
+        'workload': f'{count} independent 2D Euler four-quadrant grids, sides log-uniform in [64, 512] '
+                    f'(seed {seed}), freeflow, Roe + plm cons/minmod, RK2-TVD, CFL 0.4, FP32, {steps} steps',
+        'grids': count, 'cells': cells, 'steps': steps,
+        'solves_per_s': count / wall, 'gcell_updates_per_s': cells * steps / wall / 1e9,
+        'sequential_plain_solves_per_s': count / seq_wall,
+        'sequential_plain_gcell_updates_per_s': cells * steps / seq_wall / 1e9,
+        'speedup_vs_sequential': seq_wall / wall,
+        'launch': {'rows_per_wave': rows, 'waves_per_block': wpb, 'autotune_s': tune_s, 'graph': True},
+        'same_bits_as_plain': bool(same),
+        'fp32_rel_l1_vs_fp64_max': worst, 'fp32_rel_l1_grids': pick,
+    }
+
+
+def serving_leg(count, seed=0, steps_per_tick=20, budget=32 << 30):
+    from hydro import workloads as wl
+    from hydro.serve import serve, serve_sequential
+    reqs = wl.serving_mix(count, seed=seed)
+    cu = sum(r.cell_updates() for r in reqs)
+    lat_c, mk_c, _ = serve(reqs, budget, steps_per_tick, closed_loop=True)
+    capacity = count / mk_c
+    rate = .8 * capacity
+    reqs_p = wl.serving_mix(count, seed=seed, rate=rate)
+    lat, mk, _ = serve(reqs_p, budget, steps_per_tick)
+    lat_s, mk_s, _ = serve_sequential(reqs, closed_loop=True)
+    kinds = {k: sum(r.kind == k for r in reqs) for k in ('euler1d', 'euler2d', 'glm2d')}
+    return {
+        'workload': f'{count} solve requests (seed {seed}): 1D Euler N in [256, 65536], 2D Euler side '
+                    f'log-uniform [64, 1024], 2D GLM-Maxwell side log-uniform [128, 512] (sigma 0, eps = mu = 1); '
+                    f'100-2000 steps each, FP32, Roe + plm cons/minmod, RK2-TVD',
+        'mix': kinds, 'cell_updates': cu,
+        'closed_loop_solves_per_s': capacity, 'closed_loop_gcell_updates_per_s': cu / mk_c / 1e9,
+        'poisson_rate_per_s': rate, 'solves_per_s': count / mk,
+        'p50_latency_s': float(np.percentile(lat, 50)), 'p99_latency_s': float(np.percentile(lat, 99)),
+        'sequential_plain_solves_per_s': count / mk_s,
+        'sequential_plain_p50_latency_s': float(np.percentile(lat_s, 50)),
+        'speedup_vs_sequential': mk_s / mk_c,
+        'steps_per_tick': steps_per_tick, 'hbm_budget_bytes': budget,
+        'latency': 'arrival to result on the host; sequential latencies are closed-loop (all requests at t=0)',
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -77,40 +233,42 @@ def main():
     ap.add_argument('--real', default='double', choices=['double', 'float'])
     ap.add_argument('--size', type=int, default=N)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 KH leg')
+    ap.add_argument('--no-batch', action='store_true', help='skip the batched leg')
+    ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
     ap.add_argument('--exact-math', action='store_true',
                     help='IEEE division/sqrt (bitwise reference arithmetic) instead of the faithful fast sequences')
-    ap.add_argument('--rows', type=int, default=0, help='rows per wave of the stage kernel (0: autotuned default)')
+    ap.add_argument('--rows', type=int, default=0, help='rows per wave of the stage kernel (0: default)')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
+    ap.add_argument('--batch-grids', type=int, default=512)
+    ap.add_argument('--batch-steps', type=int, default=300)
+    ap.add_argument('--serve-requests', type=int, default=120)
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='first run one device copy of this many MiB (counter calibration for profiling)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (got --gpus {args.gpus}); the grid path is not '
               'sharded across GPUs in this build', file=sys.stderr)
         sys.exit(2)
 
-    from hydro.solver.hipfvsolver import HipFVSolver
-    from hydro.init import euler as init_euler
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device='cuda')
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     n = args.size
-    s = HipFVSolver(dim=2, gridSize=(n, n), mins=(-.5, -.5), maxs=(.5, .5), usePLM='plm cons',
-                    slopeLimiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4,
-                    boundary='periodic', real=args.real, fastMath=not args.exact_math)
-    if args.rows:
-        s.setLaunch(args.rows, 0)
-    U0 = init_euler.to_ghosted(s.cfg, init_euler.kelvin_helmholtz(s.cfg, seed=0))
-    s.resetState(U0.astype(s.dtype))
-    s.update(max(args.warmup, 1))
-    s.sync()
-
-    # timed region: exactly K steps, synchronized on both sides
-    s.sync()
-    t0 = time.perf_counter()
-    s.update(args.steps)
-    s.sync()
-    wall = time.perf_counter() - t0
+    fast = not args.exact_math
+    s = kh_solver(n, args.real, fast, args.rows)
+    wall = timed_steps(s, args.warmup, args.steps)
     cells = n * n
     value = cells * args.steps / wall / 1e9
 
     # stage kernel duration: HIP events around each launch on the solver stream
+    U_main = s.UBufObj.toCPU()
+    t_main = s.t
     kt = s.timeUpdates(min(args.steps, 100), perKernel=True)
     avg_ms = kt.stage_ms / max(kt.n_stage_launches, 1)
     w = 8 if args.real == 'double' else 4
@@ -138,7 +296,7 @@ def main():
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f64' if args.real == 'double' else 'f32',
-        'data': 'synthetic (seeded Kelvin-Helmholtz shear layer)',
+        'data': 'synthetic (seeded Kelvin-Helmholtz shear layer; seeded quadrant / Riemann / pulse states)',
         'config': {
             'workload': f'2D Euler Kelvin-Helmholtz {n}x{n} periodic, Roe + plm cons/minmod, '
                         f'RK2-TVD, CFL 0.4, {"FP64" if args.real == "double" else "FP32 bulk"}',
@@ -149,13 +307,27 @@ def main():
         'roofline': {
             'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
             'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
-            'kernel': 'hydro::stage_kernel<double, Euler, 2, PLM, minmod>' if args.real == 'double'
-                      else 'hydro::stage_kernel<float, Euler, 2, PLM, minmod>',
+            'kernel': f'hydro::stage_kernel<{args.real}, Euler, 2, plm cons, minmod>',
             'avg_launch_ms': avg_ms,
             'algorithmic_bytes_per_launch': bytes_per_launch,
         },
-        't_sim': s.t,
+        't_sim': t_main,
     }
+    del s
+
+    if not args.no_fp32 and args.real == 'double':
+        s32 = kh_solver(n, 'float', True)
+        wall32 = timed_steps(s32, args.warmup, args.steps)
+        e = rel_l1(interior(s32.UBufObj.toCPU(), 2)[..., :NQ], interior(U_main, 2)[..., :NQ])
+        out['fp32'] = {'value': cells * args.steps / wall32 / 1e9, 'unit': 'Gcell-updates/s',
+                       'rows_per_wave': s32.getLaunch()[0], 't_sim': s32.t,
+                       'rel_l1_vs_fp64': [float(x) for x in e],
+                       'note': f'after {args.warmup + args.steps} steps; each form takes its own dt'}
+        del s32
+    if not args.no_batch:
+        out['batched'] = batched_leg(args.batch_grids, args.batch_steps)
+    if not args.no_serve:
+        out['serving'] = serving_leg(args.serve_requests)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(args.cpu_seconds)
     else:

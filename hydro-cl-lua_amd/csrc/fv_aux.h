@@ -23,7 +23,7 @@ struct GridArgs {
 	int slot;
 };
 
-static __global__ void reset_slots_kernel(DevScalars *dev, int mask) {
+[[maybe_unused]] static __global__ void reset_slots_kernel(DevScalars *dev, int mask) {
 	if (blockIdx.x == 0 && threadIdx.x < DT_LANES) {
 		for (int i = 0; i < 3; ++i)
 			if (mask & (1 << i)) dev->dtmin[i][threadIdx.x] = INFINITY;
@@ -31,7 +31,7 @@ static __global__ void reset_slots_kernel(DevScalars *dev, int mask) {
 }
 
 // min of the 64 sub-slots into sub-slot 0 (for host readback)
-static __global__ void fold_slot_kernel(DevScalars *dev, int slot) {
+[[maybe_unused]] static __global__ void fold_slot_kernel(DevScalars *dev, int slot) {
 	double m = read_dt_slot(dev, slot);
 	if (threadIdx.x == 0) dev->dtmin[slot][0] = m;
 }
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(TOTALS_BLOCK) totals_partial_kernel(const R *U
 	}
 }
 
-static __global__ void __launch_bounds__(TOTALS_BLOCK) totals_final_kernel(const KSum *partial, int nPartial, int q, double *out) {
+[[maybe_unused]] static __global__ void __launch_bounds__(TOTALS_BLOCK) totals_final_kernel(const KSum *partial, int nPartial, int q, double *out) {
 	__shared__ KSum sh[TOTALS_BLOCK];
 	for (int j = 0; j < q; ++j) {
 		KSum a = {0., 0.};

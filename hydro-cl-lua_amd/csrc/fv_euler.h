@@ -24,6 +24,7 @@ struct EulerT {
 	template <typename R>
 	struct Eqn {
 		using M = Math<R, FAST>;
+		static constexpr bool FASTM = FAST;
 		static constexpr int Q = 5;       // numIntStates (euler.lua:14)
 		static constexpr int NSTAT = 0;   // no static fields on the device
 		static constexpr int NW = 5;      // numWaves (euler.lua:13)
@@ -85,31 +86,45 @@ struct EulerT {
 			const Prim &WL, const Prim &WR)
 		{
 			const R rhoEpsilon = R(1e-5);
+			// every input is read once into a value up front; the branches below
+			// only select among values (selecting between the two states'
+			// addresses would move them to scratch memory)
+			const R rhoL = UL[0], rhoR = UR[0], EL = UL[4], ER = UR[4];
+			const R PL = WL.P, PR = WR.P, invL = WL.inv, invR = WR.inv;
+			R vL[3], vR[3];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) { vL[i] = WL.v[i]; vR[i] = WR.v[i]; }
 			Eig e;
-			if (UL[0] < rhoEpsilon && UR[0] < rhoEpsilon) {
+			if (rhoL < rhoEpsilon && rhoR < rhoEpsilon) {
 				e.rho = 0; e.v[0] = e.v[1] = e.v[2] = 0; e.hTotal = 0; e.Cs = 0; e.vSq = 0;
 				return e;
 			}
-			if (UL[0] < rhoEpsilon || UR[0] < rhoEpsilon) {
-				bool useR = UL[0] < rhoEpsilon;
-				const R *Us = useR ? UR : UL;
-				const Prim &W = useR ? WR : WL;
-				e.rho = Us[0];
+			if (rhoL < rhoEpsilon || rhoR < rhoEpsilon) {
+				// the non-vacuum side
+				const bool useR = rhoL < rhoEpsilon;
+				Prim W;
+				W.rho = useR ? rhoR : rhoL;
+				W.inv = useR ? invR : invL;
+				W.P = useR ? PR : PL;
+#pragma unroll
+				for (int i = 0; i < 3; ++i) W.v[i] = useR ? vR[i] : vL[i];
+				const R ETotalS = useR ? ER : EL;
+				e.rho = W.rho;
 				e.v[0] = W.v[0]; e.v[1] = W.v[1]; e.v[2] = W.v[2];
 				e.vSq = lenSq3(W.v);
-				e.hTotal = M::div_inv(W.P + Us[4], W.rho, W.inv);
+				e.hTotal = M::div_inv(W.P + ETotalS, W.rho, W.inv);
 				e.Cs = calc_Cs(p, W);
 				return e;
 			}
-			R sqrtRhoL = M::sqrt_(WL.rho);
-			R hTotalL = M::div_inv(WL.P + UL[4], WL.rho, WL.inv);
-			R sqrtRhoR = M::sqrt_(WR.rho);
-			R hTotalR = M::div_inv(WR.P + UR[4], WR.rho, WR.inv);
+			R sqrtRhoL = M::sqrt_(rhoL);
+			R hTotalL = M::div_inv(PL + EL, rhoL, invL);
+			R sqrtRhoR = M::sqrt_(rhoR);
+			R hTotalR = M::div_inv(PR + ER, rhoR, invR);
 			R invDenom = M::rcp(sqrtRhoL + sqrtRhoR);
 			e.rho = sqrtRhoL * sqrtRhoR;
 			R wl = sqrtRhoL * invDenom, wr = sqrtRhoR * invDenom;
 #pragma unroll
-			for (int i = 0; i < 3; ++i) e.v[i] = i < NVEL ? WL.v[i] * wl + WR.v[i] * wr : R(0);
+			for (int i = 0; i < 3; ++i) e.v[i] = i < NVEL ? vL[i] * wl + vR[i] * wr : R(0);
 			R hTotal = invDenom * (sqrtRhoL * hTotalL + sqrtRhoR * hTotalR);
 			R vSq = lenSq3(e.v);
 			R eKin = R(.5) * vSq;
@@ -128,10 +143,10 @@ struct EulerT {
 		// eigen_leftTransform (euler.cl:434-489), normal along axis S
 		template <int S>
 		__device__ static __forceinline__ void left(const Par &p, const Eig &e, const R *X, R *Y) {
-			if (e.rho < p.rhoMin) {
-				for (int i = 0; i < 5; ++i) Y[i] = X[i];
-				return;
-			}
+			// identity below rhoMin: both forms are computed and selected per
+			// element, so no store into Y depends on the branch (a branch-merged
+			// store with a computed index would move Y to scratch memory)
+			const bool vac = e.rho < p.rhoMin;
 			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
 			const R Cs = e.Cs, vSq = e.vSq, g1 = p.g1;
 			R denom = R(2) * Cs * Cs;
@@ -153,37 +168,40 @@ struct EulerT {
 					y4 = y4 + X[1 + i] * cp;
 				}
 			}
-			Y[0] = (y0 + X[4] * g1) * invDenom;
-			Y[4] = (y4 + X[4] * g1) * invDenom;
+			R z[5];
+			z[0] = (y0 + X[4] * g1) * invDenom;
+			z[4] = (y4 + X[4] * g1) * invDenom;
 			R y1 = X[0] * (denom - g1 * vSq);
 #pragma unroll
 			for (int i = 0; i < 3; ++i)
 				if (i < NVEL) y1 = y1 + X[1 + i] * R(2) * g1 * e.v[i];
-			Y[1] = (y1 + X[4] * R(-2) * g1) * invDenom;
-			Y[2] = B < NVEL ? X[0] * -e.v[B] + X[1 + B] : R(0);
-			Y[3] = C < NVEL ? X[0] * -e.v[C] + X[1 + C] : R(0);
+			z[1] = (y1 + X[4] * R(-2) * g1) * invDenom;
+			z[2] = B < NVEL ? X[0] * -e.v[B] + X[1 + B] : R(0);
+			z[3] = C < NVEL ? X[0] * -e.v[C] + X[1 + C] : R(0);
+#pragma unroll
+			for (int i = 0; i < 5; ++i) Y[i] = vac ? X[i] : z[i];
 		}
 
 		// eigen_rightTransform (euler.cl:493-542)
 		template <int S>
 		__device__ static __forceinline__ void right(const Par &p, const Eig &e, const R *X, R *Y) {
-			if (e.rho < p.rhoMin) {
-				for (int i = 0; i < 5; ++i) Y[i] = X[i];
-				return;
-			}
+			const bool vac = e.rho < p.rhoMin;   // identity: selected per element, as in left
 			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
 			const R Cs = e.Cs;
-			Y[0] = X[0] + X[1] + X[4];
-			Y[1 + A] = X[0] * (e.v[A] - Cs) + X[1] * e.v[A] + X[4] * (e.v[A] + Cs);
-			if constexpr (B < NVEL) Y[1 + B] = X[0] * e.v[B] + X[1] * e.v[B] + X[2] + X[4] * e.v[B];
-			else Y[1 + B] = R(0);
-			if constexpr (C < NVEL) Y[1 + C] = X[0] * e.v[C] + X[1] * e.v[C] + X[3] + X[4] * e.v[C];
-			else Y[1 + C] = R(0);
+			R z[5];
+			z[0] = X[0] + X[1] + X[4];
+			z[1 + A] = X[0] * (e.v[A] - Cs) + X[1] * e.v[A] + X[4] * (e.v[A] + Cs);
+			if constexpr (B < NVEL) z[1 + B] = X[0] * e.v[B] + X[1] * e.v[B] + X[2] + X[4] * e.v[B];
+			else z[1 + B] = R(0);
+			if constexpr (C < NVEL) z[1 + C] = X[0] * e.v[C] + X[1] * e.v[C] + X[3] + X[4] * e.v[C];
+			else z[1 + C] = R(0);
 			R Csvn = Cs * e.v[A];
 			R y4 = X[0] * (e.hTotal - Csvn) + X[1] * R(.5) * e.vSq;
 			if constexpr (B < NVEL) y4 = y4 + X[2] * e.v[B];
 			if constexpr (C < NVEL) y4 = y4 + X[3] * e.v[C];
-			Y[4] = y4 + X[4] * (e.hTotal + Csvn);
+			z[4] = y4 + X[4] * (e.hTotal + Csvn);
+#pragma unroll
+			for (int i = 0; i < 5; ++i) Y[i] = vac ? X[i] : z[i];
 		}
 
 		// fluxFromCons (euler.cl:273-291) given the state's prim

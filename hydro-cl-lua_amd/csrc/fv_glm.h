@@ -19,6 +19,7 @@ struct GLMT {
 template <typename R>
 struct Eqn {
 	using M = Math<R, FAST>;
+	static constexpr bool FASTM = FAST;
 	static constexpr int Q = 8;
 	static constexpr int NSTAT = 4;
 	static constexpr int NW = 8;

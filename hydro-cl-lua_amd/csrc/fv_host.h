@@ -172,13 +172,31 @@ static int validate(const hydro_cfg *c) {
 	return 0;
 }
 
-static int default_rows(int dim, int nx, int ny, long targetWaves) {
-	// enough waves to fill the device without letting the 2-3 row warm-up of
-	// the y pipeline dominate
-	if (dim < 2) return 1;
+// Rows per wave of the plain stage kernel. Each wave marches `rows` rows of a
+// 60-column strip after a warm-up of 1-3 rows, so short strips waste work on
+// warm-up and long ones leave SIMDs idle in the last wave round. FP64 runs at
+// 2-3 resident waves per SIMD (VGPR-bound): pick the rows that minimise the
+// busiest SIMD's rows, ceil(waves / SIMDs) * (rows + warm-up), among launches
+// of at least 1.5 waves per SIMD (one lone wave cannot hide its own memory
+// latency). FP32 holds
+// twice as many waves, which hide latency better with more, shorter strips.
+static int default_rows(const hydro_cfg &cfg, int nx, int ny) {
+	if (cfg.dim < 2) return 1;
+	const long simds = 1024;      // 256 CUs x 4 SIMDs
 	long stripsX = (nx + STRIP_OUT - 1) / STRIP_OUT;
-	long rows = (stripsX * (long)ny + targetWaves - 1) / targetWaves;
-	return (int)std::max(4L, std::min(64L, rows));
+	if (cfg.precision == HYDRO_REAL_F32) {
+		long rows = (stripsX * (long)ny + 4607) / 4608;
+		return (int)std::max(4L, std::min(64L, rows));
+	}
+	int warm = cfg.plm ? 2 : (cfg.flux_limiter ? 3 : 1);
+	long best = -1, bestCost = 0;
+	for (long r = 4; r <= 64; ++r) {
+		long waves = stripsX * ((ny + r - 1) / r);
+		if (r > 4 && 2 * waves < 3 * simds) continue;
+		long cost = (waves + simds - 1) / simds * (r + warm);
+		if (best < 0 || cost < bestCost) { best = r; bestCost = cost; }
+	}
+	return (int)best;
 }
 
 static void grid_init(Grid &g, const hydro_cfg &cfg) {
@@ -202,7 +220,7 @@ static void grid_init(Grid &g, const hydro_cfg &cfg) {
 	g.slot = 0;
 	g.dtValid = 0;
 	g.nvel = 3;
-	g.rowsPerWave = default_rows(g.dim, g.n[0], g.n[1], 4096);
+	g.rowsPerWave = default_rows(cfg, g.n[0], g.n[1]);
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -529,7 +547,7 @@ static int grid_dt_kernel(Grid &g, int slot, hipStream_t s) {
 	return 0;
 }
 
-static int grid_constrain(Grid &g, hipStream_t s) {
+[[maybe_unused]] static int grid_constrain(Grid &g, hipStream_t s) {
 	hipError_t e = g.elem == 8 ? launch_constrain<double>(g.cfg.eqn, g.cfg.fast_math, grid_args<double>(g, 0), s)
 	                           : launch_constrain<float>(g.cfg.eqn, g.cfg.fast_math, grid_args<float>(g, 0), s);
 	if (e != hipSuccess) return fail("constrainU kernel: %s", hipGetErrorString(e));
@@ -537,7 +555,7 @@ static int grid_constrain(Grid &g, hipStream_t s) {
 	return 0;
 }
 
-static int grid_calc_dt(Grid &g, double *dt, hipStream_t s) {
+[[maybe_unused]] static int grid_calc_dt(Grid &g, double *dt, hipStream_t s) {
 	if (g.cfg.use_fixed_dt) { *dt = g.cfg.fixed_dt; return 0; }
 	if (grid_dt_kernel(g, g.slot, s)) return -1;
 	HIPCHECK(launch_fold_slot(g.dev, g.slot, s));
@@ -558,7 +576,7 @@ static int grid_get_time(const Grid &g, double *t, double *dt, hipStream_t s) {
 	return 0;
 }
 
-static int grid_set_time(Grid &g, double t, hipStream_t s) {
+[[maybe_unused]] static int grid_set_time(Grid &g, double t, hipStream_t s) {
 	HIPCHECK(hipMemcpyAsync(&g.dev->t, &t, sizeof t, hipMemcpyHostToDevice, s));
 	HIPCHECK(hipStreamSynchronize(s));
 	return 0;

@@ -8,6 +8,7 @@
 #pragma once
 
 #include "fv_common.h"
+#include "fv_math.h"
 
 namespace hydro {
 
@@ -46,6 +47,35 @@ __device__ __forceinline__ R limiter(int index, R r) {
 	else if constexpr (LIM == HYDRO_LIMITER_VAN_LEER) return r > R(0) ? r * R(2) / (R(1) + r) : R(0);
 	else if constexpr (LIM == HYDRO_LIMITER_MONOTIZED_CENTRAL) return fmax(R(0), fmin(R(2), fmin(R(.5) * (R(1) + r), R(2) * r)));
 	else return limiter_rt<R>(index, r);
+}
+
+// phi(num / den) * den, the limited quantity both the slope limiter
+// (plm.cl:56-75) and the Roe flux limiter (roe.cl:120-142) need, for the fast
+// arithmetic policy. For limiters that are piecewise linear in r the product
+// needs no division: with a = |num|, b = |den| and num, den of one sign,
+// phi(a / b) * b is a max/min of a, b, 2a, 2b, (a + b) / 2 carrying den's sign,
+// and 0 when the signs differ or either is 0 (r <= 0). van Leer needs one
+// division, 2ab / (a + b). Other limiters form r with the policy's division.
+// The results differ from the literal phi(num / den) * den by the rounding of
+// r * den only.
+template <int LIM, bool FAST, typename R>
+__device__ __forceinline__ R limited(int index, R num, R den) {
+	using M = Math<R, FAST>;
+	constexpr bool divFree = LIM == HYDRO_LIMITER_MINMOD || LIM == HYDRO_LIMITER_SUPERBEE
+		|| LIM == HYDRO_LIMITER_MONOTIZED_CENTRAL || LIM == HYDRO_LIMITER_VAN_LEER;
+	if constexpr (FAST && divFree) {
+		const bool same = (num > R(0) && den > R(0)) || (num < R(0) && den < R(0));
+		const R a = fabs(num), b = fabs(den);
+		R m;
+		if constexpr (LIM == HYDRO_LIMITER_MINMOD) m = fmin(a, b);
+		else if constexpr (LIM == HYDRO_LIMITER_SUPERBEE) m = fmax(fmin(b, R(2) * a), fmin(R(2) * b, a));
+		else if constexpr (LIM == HYDRO_LIMITER_MONOTIZED_CENTRAL) m = fmin(R(2) * b, fmin(R(.5) * (a + b), R(2) * a));
+		else m = M::div(R(2) * a * b, same ? a + b : R(1));
+		return same ? copysign(m, den) : R(0);
+	} else {
+		R r = den == R(0) ? R(0) : (FAST ? M::div(num, den) : num / den);
+		return limiter<LIM>(index, r) * den;
+	}
 }
 
 } // namespace hydro

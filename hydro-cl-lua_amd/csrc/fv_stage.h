@@ -78,8 +78,7 @@ __device__ __forceinline__ void plm_cons(int lim, const Cell<R, E> &Um, const Ce
 			bool pos = dUC >= R(0);
 			R num = pos ? dUL : dUR;
 			R den = pos ? dUR : dUL;
-			R r = den == R(0) ? R(0) : (num / den);
-			R sigma = limiter<SLIM>(lim, r) * den;
+			R sigma = limited<SLIM, E::FASTM>(lim, num, den);
 			Rr.v[j] = U.v[j] + R(.5) * sigma;
 			L.v[j] = U.v[j] - R(.5) * sigma;
 		} else {
@@ -129,7 +128,12 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 	for (int j = 0; j < E::NW; ++j) {
 		R lambda = I.lambda[j];
 		R sgnLambda = lambda >= R(0) ? R(1) : R(-1);
-		if constexpr (LIM) {
+		if constexpr (LIM && E::FASTM) {
+			// -lambda/2 (dW sgn + phi(r) dW (lambda dt/dx - sgn)), phi(r) dW without r
+			R up = lambda >= R(0) ? dWL[j] : dWR[j];
+			R phidW = limited<FLIM, true>(lim, up, I.dW[j]);
+			fluxEig[j] = R(0) - R(.5) * lambda * (I.dW[j] * sgnLambda + phidW * (lambda * dt_dx - sgnLambda));
+		} else if constexpr (LIM) {
 			R rEig;
 			if (I.dW[j] == R(0)) rEig = R(0);
 			else rEig = lambda >= R(0) ? dWL[j] / I.dW[j] : dWR[j] / I.dW[j];

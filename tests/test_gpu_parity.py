@@ -220,6 +220,15 @@ def test_flux_limiter_fast_math(lim, integ):
     _check(s, o, 2, TOL64, 20, f'fast-{lim}')
 
 
+@pytest.mark.parametrize('slim', ['superbee', 'van Leer', 'monotized central'])
+def test_plm_slope_limiters_fast_math(slim):
+    """the fast policy forms phi(r) * dU without r for these limiters"""
+    kw = dict(dim=2, gridSize=(64, 40), usePLM='plm cons', slopeLimiter=slim, integrator='Runge-Kutta 2, TVD',
+              cfl=.4, boundary='freeflow', fastMath=True)
+    s, o = _pair(_quad(13), 25, **kw)
+    _check(s, o, 2, TOL64, 25, f'fast-plm-{slim}')
+
+
 def test_sod_fast_math():
     left = [1., 0, 0, 0, 2.5, 0]
     right = [.125, 0, 0, 0, .25, 0]

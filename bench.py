@@ -8,7 +8,8 @@ exactly --steps steps between two device synchronizations.
 
 Beside it on the same JSON line:
   roofline      the stage kernel: algorithmic bytes per launch / average launch
-                duration (HIP events around each launch on the solver stream)
+                duration (HIP events on the solver stream around the timed
+                region, which holds only stage launches, over their count)
   fp32          the same KH problem in the FP32 bulk form, and its relative L1
                 difference per conserved variable from the FP64 result
   batched       512 independent 2D Euler four-quadrant grids (sides log-uniform
@@ -114,12 +115,14 @@ def kh_solver(n, real, fast, rows=0):
 
 
 def timed_steps(s, warmup, steps):
+    """wall time of exactly `steps` steps between two device synchronizations,
+    and the HIP-event time of the same region on the solver stream"""
     s.update(max(warmup, 1))
     s.sync()
     t0 = time.perf_counter()
-    s.update(steps)
+    kt = s.timeUpdates(steps, perKernel=False)     # events on the solver stream around update(steps)
     s.sync()
-    return time.perf_counter() - t0
+    return time.perf_counter() - t0, kt.total_ms
 
 
 def batched_leg(count, steps, seed=0, accuracy_grids=8):
@@ -262,15 +265,16 @@ def main():
     n = args.size
     fast = not args.exact_math
     s = kh_solver(n, args.real, fast, args.rows)
-    wall = timed_steps(s, args.warmup, args.steps)
+    wall, ev_ms = timed_steps(s, args.warmup, args.steps)
     cells = n * n
     value = cells * args.steps / wall / 1e9
 
-    # stage kernel duration: HIP events around each launch on the solver stream
+    # stage kernel duration: the timed region holds only stage launches (two
+    # per step, dt fused into the last), so the HIP-event time of the region on
+    # the solver stream over the launch count is the average launch duration
     U_main = s.UBufObj.toCPU()
     t_main = s.t
-    kt = s.timeUpdates(min(args.steps, 100), perKernel=True)
-    avg_ms = kt.stage_ms / max(kt.n_stage_launches, 1)
+    avg_ms = ev_ms / (2 * args.steps)
     w = 8 if args.real == 'double' else 4
     bytes_per_update = algorithmic_bytes_per_cell_update(w, 5)
     bytes_per_launch = bytes_per_update / 2 * cells      # two stage launches per step
@@ -317,9 +321,11 @@ def main():
 
     if not args.no_fp32 and args.real == 'double':
         s32 = kh_solver(n, 'float', True)
-        wall32 = timed_steps(s32, args.warmup, args.steps)
+        wall32, ev32 = timed_steps(s32, args.warmup, args.steps)
         e = rel_l1(interior(s32.UBufObj.toCPU(), 2)[..., :NQ], interior(U_main, 2)[..., :NQ])
         out['fp32'] = {'value': cells * args.steps / wall32 / 1e9, 'unit': 'Gcell-updates/s',
+                       'avg_launch_ms': ev32 / (2 * args.steps),
+                       'roofline_frac': 5 * NQ * 4 / 2 * cells / (ev32 / (2 * args.steps) * 1e-3) / 1e9 / HBM_PEAK_GBS,
                        'rows_per_wave': s32.getLaunch()[0], 't_sim': s32.t,
                        'rel_l1_vs_fp64': [float(x) for x in e],
                        'note': f'after {args.warmup + args.steps} steps; each form takes its own dt'}

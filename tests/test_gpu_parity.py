@@ -109,8 +109,9 @@ def test_quadrant_2d_flux_limiter(lim):
     _check(s, o, 2, TOL64, 10, f'fluxlim-{lim}')
 
 
-@pytest.mark.parametrize('integ', ['forward Euler', 'Runge-Kutta 4', 'Runge-Kutta 2 Heun',
-                                   'Runge-Kutta 4, TVD', 'Runge-Kutta 3'])
+@pytest.mark.parametrize('integ', ['forward Euler', 'Runge-Kutta 2', 'Runge-Kutta 2 Heun', 'Runge-Kutta 2 Ralston',
+                                   'Runge-Kutta 3', 'Runge-Kutta 4', 'Runge-Kutta 4, 3/8ths rule',
+                                   'Runge-Kutta 2, non-TVD', 'Runge-Kutta 4, TVD', 'Runge-Kutta 4, non-TVD'])
 def test_integrators(integ):
     kw = dict(dim=2, gridSize=(40, 40), usePLM='plm cons', slopeLimiter='monotized central',
               integrator=integ, cfl=.3, boundary='periodic', mins=(-.5, -.5), maxs=(.5, .5))

@@ -17,6 +17,8 @@ Beside it on the same JSON line:
                 per stage over all grids, hipGraph replay, autotuned launch
                 shape; solves/s against the same grids advanced one after the
                 other by the plain solver, and the FP32 error against FP64
+  blast3d       3D Euler blast 256^3, van Leer flux limiter, RK3-TVD, FP32:
+                throughput, mass / energy drift and L1 against the FP64 run
   serving       Poisson stream of mixed 1D Euler / 2D Euler / 2D GLM-Maxwell
                 solves at 80% of the measured closed-loop capacity: solves/s,
                 p50 / p99 latency, against one-at-a-time plain solves
@@ -200,6 +202,52 @@ def batched_leg(count, steps, seed=0, accuracy_grids=8):
     }
 
 
+def blast3d_leg(n, steps, seed=0):
+    """3D Euler spherical blast, n^3 freeflow on [-1, 1]^3, Roe + van Leer
+    flux limiter, RK3-TVD, CFL 0.3: FP32 bulk throughput, mass and energy
+    drift (FP64 compensated totals), and the FP32 L1 difference from the same
+    run in FP64"""
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as init_euler
+
+    def solver(real):
+        s = HipFVSolver(dim=3, gridSize=(n, n, n), mins=(-1, -1, -1), maxs=(1, 1, 1), fluxLimiter='van Leer',
+                        integrator='Runge-Kutta 3, TVD', cfl=.3, boundary='freeflow', real=real, fastMath=True)
+        U0 = init_euler.to_ghosted(s.cfg, init_euler.sphere_blast(s.cfg, seed=seed))
+        s.resetState(U0.astype(s.dtype))
+        return s
+
+    s = solver('float')
+    tot0 = s.conservedTotals()
+    s.sync()
+    t0 = time.perf_counter()
+    kt = s.timeUpdates(steps, perKernel=False)
+    s.sync()
+    wall = time.perf_counter() - t0
+    tot1 = s.conservedTotals()
+    U32 = interior(s.UBufObj.toCPU(), 3)[..., :NQ]
+    t32, rows = s.t, s.getLaunch()[0]
+    del s
+    cells = n ** 3
+    avg_ms = kt.total_ms / (3 * steps)
+    alg = 8 * NQ * 4 / 3 * cells          # RK3-TVD: 8 q w per cell-update over 3 stage launches
+    s64 = solver('double')
+    s64.update(steps)
+    U64 = interior(s64.UBufObj.toCPU(), 3)[..., :NQ]
+    t64 = s64.t
+    del s64
+    e = rel_l1(U32, U64)
+    return {
+        'workload': f'3D Euler spherical blast {n}^3 freeflow on [-1,1]^3 (P 10 in r < .1 about a seeded centre, '
+                    f'rho 1, P .1 outside), Roe + van Leer flux limiter, RK3-TVD, CFL 0.3, FP32 bulk, {steps} steps',
+        'value': cells * steps / wall / 1e9, 'unit': 'Gcell-updates/s', 'rows_per_block_chunk': rows,
+        'avg_launch_ms': avg_ms, 'roofline_frac': alg / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        'mass_drift': float((tot1[0] - tot0[0]) / tot0[0]), 'energy_drift': float((tot1[4] - tot0[4]) / tot0[4]),
+        'rel_l1_vs_fp64': [float(x) for x in e], 't_fp32': t32, 't_fp64': t64,
+        'note': 'drift includes the fluxes through the freeflow faces; each precision takes its own dt',
+    }
+
+
 def serving_leg(count, seed=0, steps_per_tick=20, budget=32 << 30):
     from hydro import workloads as wl
     from hydro.serve import serve, serve_sequential
@@ -239,6 +287,9 @@ def main():
     ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 KH leg')
     ap.add_argument('--no-batch', action='store_true', help='skip the batched leg')
     ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
+    ap.add_argument('--no-3d', action='store_true', help='skip the 3D blast leg')
+    ap.add_argument('--size3d', type=int, default=256)
+    ap.add_argument('--steps3d', type=int, default=200)
     ap.add_argument('--exact-math', action='store_true',
                     help='IEEE division/sqrt (bitwise reference arithmetic) instead of the faithful fast sequences')
     ap.add_argument('--rows', type=int, default=0, help='rows per wave of the stage kernel (0: default)')
@@ -332,6 +383,8 @@ def main():
         del s32
     if not args.no_batch:
         out['batched'] = batched_leg(args.batch_grids, args.batch_steps)
+    if not args.no_3d:
+        out['blast3d'] = blast3d_leg(args.size3d, args.steps3d)
     if not args.no_serve:
         out['serving'] = serving_leg(args.serve_requests)
     if not args.no_cpu:

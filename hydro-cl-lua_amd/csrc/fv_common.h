@@ -14,6 +14,13 @@ constexpr int NG = 2;              // numGhost (gridsolver.lua:41)
 constexpr int WAVE = 64;           // CDNA wavefront
 constexpr int STRIP_OUT = WAVE - 4;// output columns per wave (2-cell halo each side)
 
+// 3D stage kernel: waves per block (rows of a strip that share one plane
+// through LDS); FP64 halves it to keep the block's LDS near 50 KB.
+template <typename R>
+struct Block3 {
+	static constexpr int W = sizeof(R) == 4 ? 8 : 4;
+};
+
 // Device-resident per-solver scalars: dt minima rotate through three slots so
 // a step can read one, reduce into the next and clear the third.
 constexpr int DT_LANES = 64;       // dt minima are spread over 64 addresses
@@ -51,13 +58,13 @@ struct GridDesc {
 	R invdx[3];               // 1 / dx in the stored precision (glm-maxwell.cl:351)
 	double dxD[3];
 	double cfl;
-	int stripsX, stripsY, rowsPerWave;
+	int stripsX, stripsY, rowsPerWave;   // 3D: stripsY counts blocks of Block3::W rows, rowsPerWave planes per chunk
 	int active;               // batch: grid takes part in the launch
 	long itemBase;            // batch: first work item (wave strip) of this grid
 	long nItems;              // stripsX * stripsY
 	DevScalars *dev;          // dt slots and time of this grid
 	int stepsLeft;            // batch: steps still to run in the current tick
-	int pad;
+	int stripsZ;              // 3D: plane chunks of rowsPerWave planes
 };
 
 // Uniform for one launch: the stage's row of the Shu-Osher table (as buffer

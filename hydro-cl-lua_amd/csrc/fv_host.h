@@ -152,7 +152,8 @@ static size_t elem_size(const hydro_cfg &c) { return c.precision == HYDRO_REAL_F
 
 static int validate(const hydro_cfg *c) {
 	if (!c) return fail("cfg is NULL");
-	if (c->dim < 1 || c->dim > 2) return fail("dim %d not supported (1 or 2)", c->dim);
+	if (c->dim < 1 || c->dim > 3) return fail("dim %d not supported (1, 2 or 3)", c->dim);
+	if (c->dim == 3 && c->eqn != HYDRO_EQN_EULER) return fail("3D is built for the Euler equations only");
 	for (int i = 0; i < c->dim; ++i)
 		if (c->size[i] < 1) return fail("size[%d] = %d must be >= 1", i, c->size[i]);
 	if (c->eqn != HYDRO_EQN_EULER && c->eqn != HYDRO_EQN_GLM_MAXWELL) return fail("eqn %d not supported", c->eqn);
@@ -180,10 +181,26 @@ static int validate(const hydro_cfg *c) {
 // of at least 1.5 waves per SIMD (one lone wave cannot hide its own memory
 // latency). FP32 holds
 // twice as many waves, which hide latency better with more, shorter strips.
-static int default_rows(const hydro_cfg &cfg, int nx, int ny) {
+// 3D: planes per block chunk, balancing the last round of blocks (about two
+// resident blocks per CU) against the 1-3 plane warm-up of each chunk.
+static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 	if (cfg.dim < 2) return 1;
 	const long simds = 1024;      // 256 CUs x 4 SIMDs
 	long stripsX = (nx + STRIP_OUT - 1) / STRIP_OUT;
+	if (cfg.dim == 3) {
+		const long slots = 512;
+		long W = cfg.precision == HYDRO_REAL_F32 ? Block3<float>::W : Block3<double>::W;
+		long perPlane = stripsX * ((ny + W - 1) / W);
+		int warm = cfg.plm ? 2 : (cfg.flux_limiter ? 3 : 1);
+		long best = -1, bestCost = 0;
+		for (long r = 4; r <= 256; ++r) {
+			long blocks = perPlane * ((nz + r - 1) / r);
+			if (r > 4 && 2 * blocks < 3 * slots) continue;
+			long cost = (blocks + slots - 1) / slots * (r + warm);
+			if (best < 0 || cost < bestCost) { best = r; bestCost = cost; }
+		}
+		return (int)best;
+	}
 	if (cfg.precision == HYDRO_REAL_F32) {
 		long rows = (stripsX * (long)ny + 4607) / 4608;
 		return (int)std::max(4L, std::min(64L, rows));
@@ -220,7 +237,7 @@ static void grid_init(Grid &g, const hydro_cfg &cfg) {
 	g.slot = 0;
 	g.dtValid = 0;
 	g.nvel = 3;
-	g.rowsPerWave = default_rows(cfg, g.n[0], g.n[1]);
+	g.rowsPerWave = default_rows(cfg, g.n[0], g.n[1], g.n[2]);
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -313,8 +330,14 @@ static void grid_desc(const Grid &g, GridDesc<R> &d) {
 	d.cfl = c.cfl;
 	d.rowsPerWave = g.rowsPerWave;
 	d.stripsX = (g.n[0] + STRIP_OUT - 1) / STRIP_OUT;
-	d.stripsY = g.dim >= 2 ? (g.n[1] + g.rowsPerWave - 1) / g.rowsPerWave : 1;
-	d.nItems = (long)d.stripsX * d.stripsY;
+	if (g.dim == 3) {
+		d.stripsY = (g.n[1] + Block3<R>::W - 1) / Block3<R>::W;
+		d.stripsZ = (g.n[2] + g.rowsPerWave - 1) / g.rowsPerWave;
+	} else {
+		d.stripsY = g.dim >= 2 ? (g.n[1] + g.rowsPerWave - 1) / g.rowsPerWave : 1;
+		d.stripsZ = 1;
+	}
+	d.nItems = (long)d.stripsX * d.stripsY * d.stripsZ;
 	d.active = 1;
 	d.itemBase = 0;
 	d.dev = g.dev;

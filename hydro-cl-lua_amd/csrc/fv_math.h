@@ -9,8 +9,8 @@
 // correctly rounded results on 4M random operands on gfx950. Where the divisor
 // is known to be a positive O(1) quantity (densities, sound speeds) the
 // reciprocal is shared between divisions. FP32: v_rcp_f32 / v_sqrt_f32.
-// General divisions (limiter ratios) fall back to IEEE division when any lane
-// of the wave has a zero, huge, tiny or non-finite divisor.
+// General divisions (limiter ratios, FP64 and FP32) fall back to IEEE division
+// when any lane of the wave has a zero, huge, tiny or non-finite divisor.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -84,7 +84,15 @@ struct Math<float, false> {
 template <>
 struct Math<float, true> {
 	__device__ static __forceinline__ float rcp(float y) { return __builtin_amdgcn_rcpf(y); }
-	__device__ static __forceinline__ float div(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+	// v_rcp_f32 flushes a denormal divisor to an infinite reciprocal (0 * inf
+	// = NaN for x -> 0): such waves take IEEE division, as the FP64 form does
+	__device__ static __forceinline__ float div(float x, float y) {
+		float q = x * __builtin_amdgcn_rcpf(y);
+		float ay = __builtin_fabsf(y);
+		bool ok = ay > 0x1p-100f && ay < 0x1p+100f;
+		if (!__all(ok)) q = ok ? q : x / y;
+		return q;
+	}
 	__device__ static __forceinline__ float div_inv(float x, float, float inv) { return x * inv; }
 	__device__ static __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
 };

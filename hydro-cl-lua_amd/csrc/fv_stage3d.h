@@ -1,0 +1,420 @@
+// fv_stage3d.h — the fused finite-volume stage kernel in 3D (Euler).
+//
+// Same stage as the 1D/2D kernel (calcLR -> calcFlux -> calcDerivFromFlux ->
+// multAdd... -> constrainU, and the new state's dt on the last stage;
+// fvsolver.lua:225-302, rk.lua:152-183, calcDT.cl:38-73), with a third axis.
+//
+// Work decomposition: a block of W waves owns a 60-column strip (x) of W
+// consecutive rows (y) and marches a chunk of planes in z. In every wave:
+//   x neighbours: DPP lane shifts (as in 2D);
+//   z neighbours: a sliding register window of planes (as y in 2D);
+//   y neighbours: the block's rows of the current plane, exchanged through
+//     LDS. Each interface in y is computed once per block: wave w takes the
+//     interface below its row; the interfaces outside the block (one above,
+//     and for the flux limiter the characteristic jumps one further out on
+//     each side) are taken by the edge waves. Waves that wait at the barriers
+//     leave their SIMD to waves of other blocks.
+// Three barriers per plane; every wave of a block runs the same planes, so
+// all of them reach every barrier.
+#pragma once
+
+#include "fv_stage.h"
+
+namespace hydro {
+
+template <typename R, class E>
+__device__ __forceinline__ void load_cell3(const GridDesc<R> &g, const R *base, const Mapped &mx, const Mapped &my,
+	const Mapped &mz, Cell<R, E> &c)
+{
+	long off = ((long)mz.idx * g.n[1] + my.idx) * g.step1 + mx.idx;
+	int fixedFace = -1, lastFixed = -1;
+	if (mx.fixedFace >= 0) { fixedFace = mx.fixedFace; lastFixed = 0; }
+	if (my.fixedFace >= 0) { fixedFace = my.fixedFace; lastFixed = 1; }
+	if (mz.fixedFace >= 0) { fixedFace = mz.fixedFace; lastFixed = 2; }
+#pragma unroll
+	for (int f = 0; f < Cell<R, E>::NV; ++f) {
+		if (!E::active(f)) {
+			c.v[f] = R(0);
+			continue;
+		}
+		// axis by axis, a later axis wins (corners never enter the axis-split
+		// stencil): the last fixed axis gives the value, and mirrors of the
+		// axes after it flip it. One table lookup, from a face index chosen by
+		// value (selecting between lookups would copy the argument block to
+		// scratch memory).
+		const int face = fixedFace;
+		R v = base[f * g.ps + off];
+		R fv = g.fixedv[face < 0 ? 0 : face][f];
+		v = face >= 0 ? fv : v;
+		if (lastFixed < 0 && mx.mirror && E::reflects(f, 0)) v = -v;
+		if (lastFixed < 1 && my.mirror && E::reflects(f, 1)) v = -v;
+		if (lastFixed < 2 && mz.mirror && E::reflects(f, 2)) v = -v;
+		c.v[f] = v;
+	}
+}
+
+// LDS of one block: the block's rows of the current plane plus two halo rows
+// on each side, one slot per y interface quantity, and the y fluxes.
+template <typename R, class E, int W>
+struct Lds3 {
+	static constexpr int NV = Cell<R, E>::NV;
+	R u[W + 4][NV][WAVE];      // U of rows y0-2 .. y0+W+1
+	R e[W + 3][NV][WAVE];      // PLM: R edges of rows y0-1 .. y0+W-1; flux limiter: dW at y0-3/2 .. y0+W+1/2
+	R f[W + 1][E::Q][WAVE];    // area-weighted y fluxes at y0-1/2 .. y0+W-1/2
+};
+
+template <typename R, class E, int NV>
+__device__ __forceinline__ void lds_put(R (&s)[NV][WAVE], const R *v, int lane) {
+#pragma unroll
+	for (int j = 0; j < NV; ++j) s[j][lane] = v[j];
+}
+template <typename R, class E, int NV>
+__device__ __forceinline__ void lds_get(const R (&s)[NV][WAVE], R *v, int lane) {
+#pragma unroll
+	for (int j = 0; j < NV; ++j) v[j] = s[j][lane];
+}
+
+template <typename R, template <typename> class EQ, bool PLM, bool LIM, int FLIM, int SLIM, int W>
+__device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const GridDesc<R> &g, int sx, int by, int bz,
+	int w, double dt, int lane, Lds3<R, EQ<R>, W> &sh)
+{
+	using E = EQ<R>;
+	using C = Cell<R, E>;
+	constexpr int Q = E::Q;
+	constexpr int NV = C::NV;
+	static_assert(!E::SOURCE, "3D is built for the Euler equations");
+	static_assert(E::NW == NV, "dW shares the LDS slots of the states");
+	const typename E::Par par = E::par(a);
+	const int lim = PLM ? 0 : a.limiter;
+	const int slim = a.limiter;
+	double dtLocal = INFINITY;
+
+	const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+	const R *src = g.U[a.src];
+	R *out = a.out >= 0 ? g.U[a.out] : g.ping;
+	R *storeD = a.storeD >= 0 ? g.D[a.storeD] : nullptr;
+	const int x = sx * STRIP_OUT - NG + lane;
+	const int y0 = by * W;
+	const int y = y0 + w;
+	const bool outLane = lane >= NG && lane < WAVE - NG && x < nx && y < ny;
+	const Mapped mx = map_coord(x, nx, g.bc[0], g.bc[1], 0);
+	const Mapped myOwn = map_coord(y, ny, g.bc[2], g.bc[3], 2);
+	const R rdt = R(dt);
+	const R dt_dx0 = rdt / g.dx[0];
+	const R dt_dx1 = rdt / g.dx[1];
+	const R dt_dx2 = rdt / g.dx[2];
+	const int z0 = bz * g.rowsPerWave;
+	const int z1 = min(z0 + g.rowsPerWave, nz);
+
+	auto loadAt = [&](int yy, int zz, C &c) {
+		Mapped my = map_coord(yy, ny, g.bc[2], g.bc[3], 2);
+		Mapped mz = map_coord(zz, nz, g.bc[4], g.bc[5], 4);
+		load_cell3<R, E>(g, src, mx, my, mz, c);
+	};
+	auto loadPlane = [&](int zz, C &c) {
+		Mapped mz = map_coord(zz, nz, g.bc[4], g.bc[5], 4);
+		load_cell3<R, E>(g, src, mx, myOwn, mz, c);
+	};
+
+	// z pipeline state (the 2D kernel's y pipeline)
+	C W0, W1, W2;
+	C Rz0;
+	Iface<R, E> I0, I1;
+	R FaZprev[Q];
+#pragma unroll
+	for (int f = 0; f < NV; ++f) { W0.v[f] = W1.v[f] = W2.v[f] = Rz0.v[f] = R(0); }
+#pragma unroll
+	for (int j = 0; j < Q; ++j) FaZprev[j] = R(0);
+	I0 = Iface<R, E>{};
+	I1 = Iface<R, E>{};
+
+	constexpr int WARM = PLM ? 2 : (LIM ? 3 : 1);
+	if constexpr (PLM) { loadPlane(z0 - 2, W0); loadPlane(z0 - 1, W1); }
+	else if constexpr (LIM) { loadPlane(z0 - 2, W1); }
+	else { loadPlane(z0 - 1, W0); }
+
+	for (int r = z0 - WARM; r < z1; ++r) {
+		// ---- z direction: interface r+1/2, dzterm of plane r
+		R dzterm[Q];
+		{
+			R F[Q];
+			// warm-up interfaces below z0-1/2 are never used
+			const bool wantFlux = r >= z0 - 1;
+			if constexpr (PLM) {
+				loadPlane(r + 2, W2);
+				C L1, R1;
+				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
+				if (wantFlux) {
+					Iface<R, E> I;
+					iface_prep<R, E, 2>(par, Rz0, L1, I);
+					iface_flux<R, E, 2, false, 0>(par, I, nullptr, nullptr, dt_dx2, lim, F);
+				} else {
+#pragma unroll
+					for (int j = 0; j < Q; ++j) F[j] = R(0);
+				}
+				Rz0 = R1;
+			} else if constexpr (LIM) {
+				loadPlane(r + 2, W2);
+				Iface<R, E> I2;
+				iface_prep<R, E, 2>(par, W1, W2, I2);
+				if (wantFlux) iface_flux<R, E, 2, true, FLIM>(par, I1, I0.dW, I2.dW, dt_dx2, lim, F);
+				else {
+#pragma unroll
+					for (int j = 0; j < Q; ++j) F[j] = R(0);
+				}
+				I0 = I1;
+				I1 = I2;
+			} else {
+				loadPlane(r + 1, W1);
+				Iface<R, E> I;
+				iface_prep<R, E, 2>(par, W0, W1, I);
+				iface_flux<R, E, 2, false, 0>(par, I, nullptr, nullptr, dt_dx2, lim, F);
+			}
+#pragma unroll
+			for (int j = 0; j < Q; ++j) {
+				R FaZ = g.fluxOn[2] ? F[j] * g.areaOverVol[2] : R(0);
+				dzterm[j] = FaZ - FaZprev[j];
+				FaZprev[j] = FaZ;
+			}
+		}
+
+		if (r >= z0) {
+			const C &U = W0;     // plane r of this wave's row
+
+			// ---- y direction through LDS
+			lds_put<R, E, NV>(sh.u[w + 2], U.v, lane);
+			if (w == 0) {
+				C h;
+				loadAt(y0 - 2, r, h);
+				lds_put<R, E, NV>(sh.u[0], h.v, lane);
+				loadAt(y0 - 1, r, h);
+				lds_put<R, E, NV>(sh.u[1], h.v, lane);
+			}
+			if (w == W - 1) {
+				C h;
+				loadAt(y0 + W, r, h);
+				lds_put<R, E, NV>(sh.u[W + 2], h.v, lane);
+				loadAt(y0 + W + 1, r, h);
+				lds_put<R, E, NV>(sh.u[W + 3], h.v, lane);
+			}
+			__syncthreads();
+
+			R Fy[Q], FyTop[Q];
+			if constexpr (PLM) {
+				C Um, Up, Ly, Ry;
+				lds_get<R, E, NV>(sh.u[w + 1], Um.v, lane);
+				lds_get<R, E, NV>(sh.u[w + 3], Up.v, lane);
+				plm_cons<R, E, SLIM>(slim, Um, U, Up, Ly, Ry);
+				lds_put<R, E, NV>(sh.e[w + 1], Ry.v, lane);
+				C Ltop;
+				if (w == 0) {
+					C a0, a1, a2, Le, Re;
+					lds_get<R, E, NV>(sh.u[0], a0.v, lane);
+					lds_get<R, E, NV>(sh.u[1], a1.v, lane);
+					lds_get<R, E, NV>(sh.u[2], a2.v, lane);
+					plm_cons<R, E, SLIM>(slim, a0, a1, a2, Le, Re);
+					lds_put<R, E, NV>(sh.e[0], Re.v, lane);
+				}
+				if (w == W - 1) {
+					C a0, a1, a2, Re;
+					lds_get<R, E, NV>(sh.u[W + 1], a0.v, lane);
+					lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
+					lds_get<R, E, NV>(sh.u[W + 3], a2.v, lane);
+					plm_cons<R, E, SLIM>(slim, a0, a1, a2, Ltop, Re);
+				}
+				__syncthreads();
+				C Rm;
+				lds_get<R, E, NV>(sh.e[w], Rm.v, lane);
+				Iface<R, E> I;
+				iface_prep<R, E, 1>(par, Rm, Ly, I);
+				iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, Fy);
+				if (w == W - 1) {
+					Iface<R, E> It;
+					iface_prep<R, E, 1>(par, Ry, Ltop, It);
+					iface_flux<R, E, 1, false, 0>(par, It, nullptr, nullptr, dt_dx1, lim, FyTop);
+				}
+			} else if constexpr (LIM) {
+				// slot k of sh.e holds dW of interface y0 + k - 3/2
+				C Um;
+				lds_get<R, E, NV>(sh.u[w + 1], Um.v, lane);
+				Iface<R, E> I;
+				iface_prep<R, E, 1>(par, Um, U, I);
+				lds_put<R, E, NV>(sh.e[w + 1], I.dW, lane);
+				Iface<R, E> It;
+				if (w == 0) {
+					C a0, a1;
+					lds_get<R, E, NV>(sh.u[0], a0.v, lane);
+					lds_get<R, E, NV>(sh.u[1], a1.v, lane);
+					Iface<R, E> Ie;
+					iface_prep<R, E, 1>(par, a0, a1, Ie);
+					lds_put<R, E, NV>(sh.e[0], Ie.dW, lane);
+				}
+				if (w == W - 2) {
+					C a0, a1;
+					lds_get<R, E, NV>(sh.u[W + 2], a0.v, lane);
+					lds_get<R, E, NV>(sh.u[W + 3], a1.v, lane);
+					Iface<R, E> Ie;
+					iface_prep<R, E, 1>(par, a0, a1, Ie);
+					lds_put<R, E, NV>(sh.e[W + 2], Ie.dW, lane);
+				}
+				if (w == W - 1) {
+					C a1;
+					lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
+					iface_prep<R, E, 1>(par, U, a1, It);
+					lds_put<R, E, NV>(sh.e[W + 1], It.dW, lane);
+				}
+				__syncthreads();
+				R dWL[E::NW], dWR[E::NW];
+				lds_get<R, E, NV>(sh.e[w], dWL, lane);
+				lds_get<R, E, NV>(sh.e[w + 2], dWR, lane);
+				iface_flux<R, E, 1, true, FLIM>(par, I, dWL, dWR, dt_dx1, lim, Fy);
+				if (w == W - 1) {
+					lds_get<R, E, NV>(sh.e[W + 2], dWR, lane);
+					iface_flux<R, E, 1, true, FLIM>(par, It, I.dW, dWR, dt_dx1, lim, FyTop);
+				}
+			} else {
+				C Um;
+				lds_get<R, E, NV>(sh.u[w + 1], Um.v, lane);
+				Iface<R, E> I;
+				iface_prep<R, E, 1>(par, Um, U, I);
+				iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, Fy);
+				if (w == W - 1) {
+					C a1;
+					lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
+					Iface<R, E> It;
+					iface_prep<R, E, 1>(par, U, a1, It);
+					iface_flux<R, E, 1, false, 0>(par, It, nullptr, nullptr, dt_dx1, lim, FyTop);
+				}
+			}
+			R FaY[Q];
+#pragma unroll
+			for (int j = 0; j < Q; ++j) {
+				FaY[j] = g.fluxOn[1] ? Fy[j] * g.areaOverVol[1] : R(0);
+				sh.f[w][j][lane] = FaY[j];
+				if (w == W - 1) sh.f[W][j][lane] = g.fluxOn[1] ? FyTop[j] * g.areaOverVol[1] : R(0);
+			}
+			__syncthreads();
+			R dyterm[Q];
+#pragma unroll
+			for (int j = 0; j < Q; ++j) dyterm[j] = sh.f[w + 1][j][lane] - FaY[j];
+
+			// ---- x direction (lanes)
+			R Fx[Q];
+			if constexpr (PLM) {
+				C Um, Up, Lx, Rx, RxL;
+				shift_left<R, E>(U, Um);
+				shift_right<R, E>(U, Up);
+				plm_cons<R, E, SLIM>(slim, Um, U, Up, Lx, Rx);
+				shift_left<R, E>(Rx, RxL);
+				Iface<R, E> I;
+				iface_prep<R, E, 0>(par, RxL, Lx, I);
+				iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+			} else if constexpr (LIM) {
+				C Um;
+				shift_left<R, E>(U, Um);
+				Iface<R, E> I;
+				iface_prep<R, E, 0>(par, Um, U, I);
+				R dWL[E::NW], dWR[E::NW];
+#pragma unroll
+				for (int j = 0; j < E::NW; ++j) { dWL[j] = from_left(I.dW[j]); dWR[j] = from_right(I.dW[j]); }
+				iface_flux<R, E, 0, true, FLIM>(par, I, dWL, dWR, dt_dx0, lim, Fx);
+			} else {
+				C Um;
+				shift_left<R, E>(U, Um);
+				Iface<R, E> I;
+				iface_prep<R, E, 0>(par, Um, U, I);
+				iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+			}
+			R deriv[Q];
+#pragma unroll
+			for (int j = 0; j < Q; ++j) {
+				R Fa = g.fluxOn[0] ? Fx[j] * g.areaOverVol[0] : R(0);
+				R FaR = from_right(Fa);
+				deriv[j] = R(0) - (FaR - Fa);
+				deriv[j] = deriv[j] - dyterm[j];
+				deriv[j] = deriv[j] - dzterm[j];
+				if (!g.derivOn) deriv[j] = R(0);
+			}
+
+			// ---- integrator combine, constrainU, new state's dt
+			if (outLane) {
+				long off = ((long)r * ny + y) * g.step1 + x;
+				double acc[Q];
+#pragma unroll
+				for (int j = 0; j < Q; ++j) acc[j] = 0.;
+				for (int k = 0; k < a.nAlpha; ++k) {
+					const bool fromSrc = a.accU[k] == a.src;
+					const R *Uk = g.U[a.accU[k]];
+					double al = a.alpha[k];
+#pragma unroll
+					for (int j = 0; j < Q; ++j) {
+						if (!E::active(j)) continue;
+						R u = fromSrc ? U.v[j] : Uk[j * g.ps + off];
+						acc[j] = acc[j] + (double)u * al;
+					}
+				}
+				for (int k = 0; k < a.nStoredD; ++k) {
+					const R *Dk = g.D[a.accD[k]];
+					double c = (double)R(a.beta[k] * dt);
+#pragma unroll
+					for (int j = 0; j < Q; ++j)
+						if (E::active(j)) acc[j] = acc[j] + (double)Dk[j * g.ps + off] * c;
+				}
+				if (storeD) {
+#pragma unroll
+					for (int j = 0; j < Q; ++j)
+						if (E::active(j)) storeD[j * g.ps + off] = deriv[j];
+				}
+				if (a.betaNew != 0.) {
+					double c = (double)R(a.betaNew * dt);
+#pragma unroll
+					for (int j = 0; j < Q; ++j)
+						if (E::active(j)) acc[j] = acc[j] + (double)deriv[j] * c;
+				}
+				E::constrain(a.gammaD, a.rhoMinD, a.PMinD, acc);
+				R o[Q];
+#pragma unroll
+				for (int j = 0; j < Q; ++j) {
+					o[j] = R(acc[j]);
+					if (E::active(j)) out[j * g.ps + off] = o[j];
+				}
+				if (a.reduceSlot >= 0) {
+					double ud[NV];
+#pragma unroll
+					for (int j = 0; j < Q; ++j) ud[j] = (double)o[j];
+#pragma unroll
+					for (int j = Q; j < NV; ++j) ud[j] = (double)U.v[j];
+					double d = E::template dt_cell<3>(a.gammaD, a.rhoMinD, a.PMinD, g.dxD, ud, a.divPhiD, a.divPsiD);
+					dtLocal = fmin(dtLocal, d);
+				}
+			}
+		}
+
+		// ---- advance the plane window
+		if constexpr (PLM || LIM) { W0 = W1; W1 = W2; }
+		else { W0 = W1; }
+	}
+	return dtLocal;
+}
+
+// One block of W waves per (strip, row block, plane chunk); grid = items.
+template <typename R, template <typename> class EQ, bool PLM, bool LIM, int FLIM, int SLIM>
+__global__ void __launch_bounds__(WAVE * Block3<R>::W) stage_kernel3d(const StageArgs<R> a)
+{
+	constexpr int W = Block3<R>::W;
+	__shared__ Lds3<R, EQ<R>, W> sh;
+	const int lane = threadIdx.x & 63;
+	const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
+	const GridDesc<R> &g = a.g;
+	double dt = stage_dt(a, g);
+	const int sx = blk % g.stripsX;
+	const int rest = blk / g.stripsX;
+	const int by = rest % g.stripsY;
+	const int bz = rest / g.stripsY;
+	if (blk == 0 && w == 0) stage_first_strip(a, g, dt, lane);
+	double dtLocal = stage_block3d<R, EQ, PLM, LIM, FLIM, SLIM, W>(a, g, sx, by, bz, w, dt, lane, sh);
+	stage_reduce(a, g, dtLocal, (long)blk * W + w, lane);
+}
+
+} // namespace hydro

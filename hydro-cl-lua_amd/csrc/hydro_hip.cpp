@@ -146,7 +146,7 @@ static int solver_stage(hydro_solver *h, int s, int useSlot, double hostDt, int 
 	a.reduceSlot = reduceSlot;
 	a.advanceTime = advance;
 	a.xcdRemap = h->xcdRemap;
-	a.nBlocks = (int)((a.g.nItems + 3) / 4);
+	a.nBlocks = (int)(g.dim == 3 ? a.g.nItems : (a.g.nItems + 3) / 4);
 	if (ev0) HIPCHECK(hipEventRecord(ev0, h->stream));
 	hipError_t e = launch_stage<R>(stage_key(g), a, dim3(a.nBlocks), dim3(256), h->stream);
 	if (e != hipSuccess) return fail("stage kernel launch: %s", hipGetErrorString(e));
@@ -383,6 +383,7 @@ extern "C" int hydro_batch_create(const hydro_cfg *cfgs, int n, hydro_batch **ou
 	if (!cfgs || n < 1) return fail("need at least one grid");
 	for (int i = 0; i < n; ++i) {
 		if (validate(&cfgs[i])) return fail("grid %d: %s", i, hydro_err_buf().c_str());
+		if (cfgs[i].dim == 3) return fail("grid %d: the batched form runs 1D and 2D grids", i);
 		if (!same_kernel_config(cfgs[0], cfgs[i]))
 			return fail("grid %d: a batch shares eqn, dim, flux, limiters, plm, integrator, precision, math and "
 			            "equation parameters", i);

@@ -168,6 +168,7 @@ extern "C" int hydro_server_submit(hydro_server *s, const hydro_cfg *cfg, const 
 {
 	if (!s || !cfg || !U || !id) return fail("NULL argument");
 	if (validate(cfg)) return -1;
+	if (cfg->dim == 3) return fail("the server runs 1D and 2D requests");
 	RKTable t;
 	rk_table(cfg->integrator, t);
 	if (t.order < 2) return fail("the server runs the Runge-Kutta integrators (forward Euler swaps buffers every step)");

@@ -15,10 +15,12 @@ constexpr int WAVE = 64;           // CDNA wavefront
 constexpr int STRIP_OUT = WAVE - 4;// output columns per wave (2-cell halo each side)
 
 // 3D stage kernel: waves per block (rows of a strip that share one plane
-// through LDS); FP64 halves it to keep the block's LDS near 50 KB.
+// through LDS). Four, so that two blocks fit per CU at the kernel's register
+// use and one block's barrier waits are covered by the other's work (8 waves
+// per block ran 30% slower at 256^3 FP32).
 template <typename R>
 struct Block3 {
-	static constexpr int W = sizeof(R) == 4 ? 8 : 4;
+	static constexpr int W = 4;
 };
 
 // Device-resident per-solver scalars: dt minima rotate through three slots so

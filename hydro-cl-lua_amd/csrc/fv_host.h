@@ -181,23 +181,24 @@ static int validate(const hydro_cfg *c) {
 // of at least 1.5 waves per SIMD (one lone wave cannot hide its own memory
 // latency). FP32 holds
 // twice as many waves, which hide latency better with more, shorter strips.
-// 3D: planes per block chunk, balancing the last round of blocks (about two
-// resident blocks per CU) against the 1-3 plane warm-up of each chunk.
+// 3D: planes per block chunk. Blocks are small (4 waves) and many resident
+// per CU, so the tail matters less than in 2D: keep at least 4096 blocks and,
+// among those chunk sizes, the fewest plane iterations per column (chunks that
+// divide the depth evenly, each with its 1-3 plane warm-up).
 static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 	if (cfg.dim < 2) return 1;
 	const long simds = 1024;      // 256 CUs x 4 SIMDs
 	long stripsX = (nx + STRIP_OUT - 1) / STRIP_OUT;
 	if (cfg.dim == 3) {
-		const long slots = 512;
 		long W = cfg.precision == HYDRO_REAL_F32 ? Block3<float>::W : Block3<double>::W;
 		long perPlane = stripsX * ((ny + W - 1) / W);
 		int warm = cfg.plm ? 2 : (cfg.flux_limiter ? 3 : 1);
-		long best = -1, bestCost = 0;
-		for (long r = 4; r <= 256; ++r) {
-			long blocks = perPlane * ((nz + r - 1) / r);
-			if (r > 4 && 2 * blocks < 3 * slots) continue;
-			long cost = (blocks + slots - 1) / slots * (r + warm);
-			if (best < 0 || cost < bestCost) { best = r; bestCost = cost; }
+		long best = 8, bestCost = -1;
+		for (long r = 8; r <= 64; ++r) {
+			long chunks = (nz + r - 1) / r;
+			if (r > 8 && perPlane * chunks < 4096) continue;
+			long cost = chunks * (r + warm);
+			if (bestCost < 0 || cost < bestCost) { best = r; bestCost = cost; }
 		}
 		return (int)best;
 	}

@@ -184,13 +184,15 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	// y pipeline state
 	C Wm, W0, W1, W2;       // rows r-1 .. r+2
 	C Ry0;                  // PLM: R edge of row r (from the previous iteration)
-	Iface<R, E> I0, I1;     // LIM: interfaces r-1/2 and r+1/2
+	Iface<R, E> I1;         // LIM: interface r+1/2
+	R dW0[E::NW];           // LIM: dW of interface r-1/2
 	R FaYprev[Q];
 #pragma unroll
 	for (int f = 0; f < C::NV; ++f) { Wm.v[f] = W0.v[f] = W1.v[f] = W2.v[f] = Ry0.v[f] = R(0); }
 #pragma unroll
 	for (int j = 0; j < Q; ++j) FaYprev[j] = R(0);
-	I0 = Iface<R, E>{};
+#pragma unroll
+	for (int j = 0; j < E::NW; ++j) dW0[j] = R(0);
 	I1 = Iface<R, E>{};
 
 	auto loadRow = [&](int r, C &c) {
@@ -232,8 +234,9 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				loadRow(r + 2, W2);
 				Iface<R, E> I2;
 				iface_prep<R, E, 1>(par, W1, W2, I2);
-				iface_flux<R, E, 1, true, FLIM>(par, I1, I0.dW, I2.dW, dt_dx1, lim, F);
-				I0 = I1;
+				iface_flux<R, E, 1, true, FLIM>(par, I1, dW0, I2.dW, dt_dx1, lim, F);
+#pragma unroll
+				for (int j = 0; j < E::NW; ++j) dW0[j] = I1.dW[j];
 				I1 = I2;
 			} else {
 				// W0 = U(r): interface r+1/2

@@ -119,13 +119,15 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	// z pipeline state (the 2D kernel's y pipeline)
 	C W0, W1, W2;
 	C Rz0;
-	Iface<R, E> I0, I1;
+	Iface<R, E> I1;        // LIM: interface r+1/2
+	R dW0[E::NW];          // LIM: dW of interface r-1/2
 	R FaZprev[Q];
 #pragma unroll
 	for (int f = 0; f < NV; ++f) { W0.v[f] = W1.v[f] = W2.v[f] = Rz0.v[f] = R(0); }
 #pragma unroll
 	for (int j = 0; j < Q; ++j) FaZprev[j] = R(0);
-	I0 = Iface<R, E>{};
+#pragma unroll
+	for (int j = 0; j < E::NW; ++j) dW0[j] = R(0);
 	I1 = Iface<R, E>{};
 
 	constexpr int WARM = PLM ? 2 : (LIM ? 3 : 1);
@@ -157,12 +159,13 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				loadPlane(r + 2, W2);
 				Iface<R, E> I2;
 				iface_prep<R, E, 2>(par, W1, W2, I2);
-				if (wantFlux) iface_flux<R, E, 2, true, FLIM>(par, I1, I0.dW, I2.dW, dt_dx2, lim, F);
+				if (wantFlux) iface_flux<R, E, 2, true, FLIM>(par, I1, dW0, I2.dW, dt_dx2, lim, F);
 				else {
 #pragma unroll
 					for (int j = 0; j < Q; ++j) F[j] = R(0);
 				}
-				I0 = I1;
+#pragma unroll
+				for (int j = 0; j < E::NW; ++j) dW0[j] = I1.dW[j];
 				I1 = I2;
 			} else {
 				loadPlane(r + 1, W1);

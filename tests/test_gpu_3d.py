@@ -2,7 +2,7 @@
 
 FP64 (IEEE and fast) within 1e-12 relative L1 per conserved variable, FP32
 bulk within 1e-5 of the FP64 oracle. The grids are small and uneven so that
-strips, row blocks (8 rows FP32, 4 FP64) and plane chunks end inside the grid,
+strips, row blocks (4 rows) and plane chunks end inside the grid,
 and every boundary kind meets every face.
 """
 import numpy as np

@@ -70,15 +70,18 @@ struct EulerT {
 #pragma unroll
 			for (int i = 0; i < 3; ++i) W.v[i] = i < NVEL ? U[1 + i] * inv : R(0);
 			R lenSq = lenSq3(U + 1);
-			R EKin = U[0] < p.rhoMin ? R(0) : M::div_inv(R(.5) * lenSq, U[0], inv);
-			W.P = U[0] < p.rhoMin ? R(0) : (p.g1 * (U[4] - EKin));
+			R EKin = M::div_inv(R(.5) * lenSq, U[0], inv);
+			W.P = p.g1 * (U[4] - EKin);
+			// below rhoMin the reference's P is 0 (its EKin 0 only feeds P);
+			// wave-uniform test, so states above rhoMin pay no select
+			if (__any(U[0] < p.rhoMin)) W.P = U[0] < p.rhoMin ? R(0) : W.P;
 			return W;
 		}
 
 		__device__ static __forceinline__ R calc_Cs(const Par &p, const Prim &W) {
 			if (W.P <= p.PMin) return R(0);
 			if (W.rho < p.rhoMin) return R(INFINITY);
-			return M::sqrt_(M::div_inv(p.gamma * W.P, W.rho, W.inv));
+			return M::sqrt_pos(M::div_inv(p.gamma * W.P, W.rho, W.inv));   // P > PMin, rho >= rhoMin
 		}
 
 		// eigen_forInterface (euler.cl:347-430) from the two states and their prims
@@ -116,9 +119,9 @@ struct EulerT {
 				e.Cs = calc_Cs(p, W);
 				return e;
 			}
-			R sqrtRhoL = M::sqrt_(rhoL);
+			R sqrtRhoL = M::sqrt_pos(rhoL);      // both >= rhoEpsilon on this path
 			R hTotalL = M::div_inv(PL + EL, rhoL, invL);
-			R sqrtRhoR = M::sqrt_(rhoR);
+			R sqrtRhoR = M::sqrt_pos(rhoR);
 			R hTotalR = M::div_inv(PR + ER, rhoR, invR);
 			R invDenom = M::rcp(sqrtRhoL + sqrtRhoR);
 			e.rho = sqrtRhoL * sqrtRhoR;
@@ -134,7 +137,7 @@ struct EulerT {
 				e.Cs = 0;
 			} else {
 				e.hTotal = hTotal;
-				e.Cs = M::sqrt_(p.g1 * h);
+				e.Cs = M::sqrt_pos(p.g1 * h);        // h >= rhoEpsilon
 			}
 			e.vSq = vSq;
 			return e;
@@ -143,10 +146,12 @@ struct EulerT {
 		// eigen_leftTransform (euler.cl:434-489), normal along axis S
 		template <int S>
 		__device__ static __forceinline__ void left(const Par &p, const Eig &e, const R *X, R *Y) {
-			// identity below rhoMin: both forms are computed and selected per
-			// element, so no store into Y depends on the branch (a branch-merged
-			// store with a computed index would move Y to scratch memory)
+			// identity below rhoMin: both forms are computed and, when any lane
+			// of the wave is below, selected per element; every store into Y has
+			// a constant index (a branch-merged store with a computed index
+			// would move Y to scratch memory)
 			const bool vac = e.rho < p.rhoMin;
+			const bool anyVac = __any(vac);
 			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
 			const R Cs = e.Cs, vSq = e.vSq, g1 = p.g1;
 			R denom = R(2) * Cs * Cs;
@@ -178,14 +183,20 @@ struct EulerT {
 			z[1] = (y1 + X[4] * R(-2) * g1) * invDenom;
 			z[2] = B < NVEL ? X[0] * -e.v[B] + X[1 + B] : R(0);
 			z[3] = C < NVEL ? X[0] * -e.v[C] + X[1 + C] : R(0);
+			if (anyVac) {
 #pragma unroll
-			for (int i = 0; i < 5; ++i) Y[i] = vac ? X[i] : z[i];
+				for (int i = 0; i < 5; ++i) Y[i] = vac ? X[i] : z[i];
+			} else {
+#pragma unroll
+				for (int i = 0; i < 5; ++i) Y[i] = z[i];
+			}
 		}
 
 		// eigen_rightTransform (euler.cl:493-542)
 		template <int S>
 		__device__ static __forceinline__ void right(const Par &p, const Eig &e, const R *X, R *Y) {
 			const bool vac = e.rho < p.rhoMin;   // identity: selected per element, as in left
+			const bool anyVac = __any(vac);
 			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
 			const R Cs = e.Cs;
 			R z[5];
@@ -200,8 +211,13 @@ struct EulerT {
 			if constexpr (B < NVEL) y4 = y4 + X[2] * e.v[B];
 			if constexpr (C < NVEL) y4 = y4 + X[3] * e.v[C];
 			z[4] = y4 + X[4] * (e.hTotal + Csvn);
+			if (anyVac) {
 #pragma unroll
-			for (int i = 0; i < 5; ++i) Y[i] = vac ? X[i] : z[i];
+				for (int i = 0; i < 5; ++i) Y[i] = vac ? X[i] : z[i];
+			} else {
+#pragma unroll
+				for (int i = 0; i < 5; ++i) Y[i] = z[i];
+			}
 		}
 
 		// fluxFromCons (euler.cl:273-291) given the state's prim
@@ -242,7 +258,7 @@ struct EulerT {
 			double rho = U[0];
 #pragma unroll
 			for (int i = 0; i < 3; ++i) U[1 + i] = i < NVEL ? v[i] * rho : 0.;
-			U[4] = rho * (.5 * lenSq3(v)) + MD::div(P, g1);
+			U[4] = rho * (.5 * lenSq3(v)) + MD::div_pos(P, g1);
 		}
 
 		// calcDTCell (calcDT.cl:38-73, euler.lua:346-373) in double
@@ -258,7 +274,7 @@ struct EulerT {
 			double Cs;
 			if (P <= PMin) Cs = 0;
 			else if (U[0] < rhoMin) Cs = INFINITY;
-			else Cs = MD::sqrt_(MD::div_inv(gamma * P, U[0], inv));
+			else Cs = MD::sqrt_pos(MD::div_inv(gamma * P, U[0], inv));
 			double dt = INFINITY;
 #pragma unroll
 			for (int s = 0; s < DIM; ++s) {

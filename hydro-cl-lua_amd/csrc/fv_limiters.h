@@ -58,12 +58,15 @@ __device__ __forceinline__ R limiter(int index, R r) {
 // division, 2ab / (a + b). Other limiters form r with the policy's division.
 // The results differ from the literal phi(num / den) * den by the rounding of
 // r * den only.
+__host__ __device__ constexpr bool limited_symmetric(int LIM) {
+	return LIM == HYDRO_LIMITER_MINMOD || LIM == HYDRO_LIMITER_SUPERBEE
+		|| LIM == HYDRO_LIMITER_MONOTIZED_CENTRAL || LIM == HYDRO_LIMITER_VAN_LEER;
+}
+
 template <int LIM, bool FAST, typename R>
 __device__ __forceinline__ R limited(int index, R num, R den) {
 	using M = Math<R, FAST>;
-	constexpr bool divFree = LIM == HYDRO_LIMITER_MINMOD || LIM == HYDRO_LIMITER_SUPERBEE
-		|| LIM == HYDRO_LIMITER_MONOTIZED_CENTRAL || LIM == HYDRO_LIMITER_VAN_LEER;
-	if constexpr (FAST && divFree) {
+	if constexpr (FAST && limited_symmetric(LIM)) {
 		const bool same = (num > R(0) && den > R(0)) || (num < R(0) && den < R(0));
 		const R a = fabs(num), b = fabs(den);
 		R m;

@@ -27,6 +27,8 @@ struct Math<double, false> {
 	// x / y given inv = rcp(y): exact division
 	__device__ static __forceinline__ double div_inv(double x, double y, double) { return x / y; }
 	__device__ static __forceinline__ double sqrt_(double x) { return __builtin_sqrt(x); }
+	__device__ static __forceinline__ double sqrt_pos(double x) { return __builtin_sqrt(x); }
+	__device__ static __forceinline__ double div_pos(double x, double y) { return x / y; }
 };
 
 template <>
@@ -59,6 +61,17 @@ struct Math<double, true> {
 		if (!__all(ok)) q = ok ? q : x / y;
 		return q;
 	}
+	// positive, normal arguments (no guard: values outside that range are
+	// never used by the callers)
+	__device__ static __forceinline__ double sqrt_pos(double x) {
+		double y = __builtin_amdgcn_rsq(x);
+		double s = x * y, h = .5 * y;
+		double r = __builtin_fma(-s, h, .5);
+		s = __builtin_fma(s, r, s);
+		h = __builtin_fma(h, r, h);
+		double d = __builtin_fma(-s, s, x);
+		return __builtin_fma(d, h, s);
+	}
 	__device__ static __forceinline__ double sqrt_(double x) {
 		double y = __builtin_amdgcn_rsq(x);
 		double s = x * y, h = .5 * y;
@@ -79,6 +92,8 @@ struct Math<float, false> {
 	__device__ static __forceinline__ float div(float x, float y) { return x / y; }
 	__device__ static __forceinline__ float div_inv(float x, float y, float) { return x / y; }
 	__device__ static __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+	__device__ static __forceinline__ float sqrt_pos(float x) { return __builtin_sqrtf(x); }
+	__device__ static __forceinline__ float div_pos(float x, float y) { return x / y; }
 };
 
 template <>
@@ -95,6 +110,8 @@ struct Math<float, true> {
 	}
 	__device__ static __forceinline__ float div_inv(float x, float, float inv) { return x * inv; }
 	__device__ static __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+	__device__ static __forceinline__ float sqrt_pos(float x) { return __builtin_amdgcn_sqrtf(x); }
+	__device__ static __forceinline__ float div_pos(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
 };
 
 } // namespace hydro

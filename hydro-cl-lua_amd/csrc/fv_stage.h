@@ -75,10 +75,17 @@ __device__ __forceinline__ void plm_cons(int lim, const Cell<R, E> &Um, const Ce
 			R dUR = Up.v[j] - U.v[j];
 			R dUL = U.v[j] - Um.v[j];
 			R dUC = R(.5) * (dUR - dUL);
-			bool pos = dUC >= R(0);
-			R num = pos ? dUL : dUR;
-			R den = pos ? dUR : dUL;
-			R sigma = limited<SLIM, E::FASTM>(lim, num, den);
+			R sigma;
+			if constexpr (E::FASTM && limited_symmetric(SLIM)) {
+				// phi(num/den) den is symmetric in its operands for these
+				// limiters: the choice by the sign of dUC drops out
+				sigma = limited<SLIM, true>(lim, dUL, dUR);
+			} else {
+				bool pos = dUC >= R(0);
+				R num = pos ? dUL : dUR;
+				R den = pos ? dUR : dUL;
+				sigma = limited<SLIM, E::FASTM>(lim, num, den);
+			}
 			Rr.v[j] = U.v[j] + R(.5) * sigma;
 			L.v[j] = U.v[j] - R(.5) * sigma;
 		} else {
@@ -140,7 +147,9 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 			R phi = limiter<FLIM>(lim, rEig);
 			fluxEig[j] = R(0) - R(.5) * lambda * I.dW[j] * (sgnLambda + phi * (lambda * dt_dx - sgnLambda));
 		} else {
-			fluxEig[j] = R(0) - R(.5) * lambda * I.dW[j] * sgnLambda;
+			// = 0 - .5 lambda dW sgn(lambda), bit for bit: scaling by +-1 commutes
+			// with round-to-nearest, and 0 - (+-0) is +0 either way
+			fluxEig[j] = R(0) - R(.5) * fabs(lambda) * I.dW[j];
 		}
 	}
 	R Y[E::Q];
@@ -395,8 +404,16 @@ __device__ __forceinline__ void stage_reduce(const StageArgs<R> &a, const GridDe
 
 // ---------------------------------------------------------- the kernels
 // Plain form: one grid, one wave per strip.
+// Occupancy hint of the plain kernel (compile-time tuning; none by default):
+// -DHYDRO_STAGE_WPE=n asks for at least n waves per SIMD.
+#ifdef HYDRO_STAGE_WPE
+#define HYDRO_STAGE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(HYDRO_STAGE_WPE)))
+#else
+#define HYDRO_STAGE_OCCUPANCY
+#endif
+
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
-__global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
+__global__ void __launch_bounds__(256) HYDRO_STAGE_OCCUPANCY stage_kernel(const StageArgs<R> a)
 {
 	const int lane = threadIdx.x & 63;
 	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;

@@ -86,8 +86,8 @@ struct Pool {
 	std::vector<char> hostDescs;
 	hipStream_t stream = nullptr;
 	int slot = 0;
-	int rowsPerWave = 4;
-	int wavesPerBlock = 4;
+	int rowsPerWave = 12;
+	int wavesPerBlock = 1;
 };
 
 bool same_pool(const hydro_cfg &a, const hydro_cfg &b) {
@@ -121,6 +121,10 @@ struct hydro_server {
 	std::vector<long long> finished;   // not yet fetched
 	std::vector<Pool> pools;
 	hipStream_t upload = nullptr;
+	// launch shape of 2D pools: the batch autotuner's choice for an ensemble
+	// of mixed 64..512 grids (12 rows per wave, one wave per block)
+	int rowsPerWave = 12;
+	int wavesPerBlock = 1;
 };
 
 extern "C" int hydro_server_create(size_t hbm_budget_bytes, int steps_per_tick, hydro_server **out) {
@@ -196,7 +200,8 @@ static int pool_for(hydro_server *s, const Request &r) {
 	Pool p;
 	p.cfg = r.cfg;
 	p.nvel = r.g.nvel;
-	p.rowsPerWave = r.g.dim >= 2 ? 4 : 1;
+	p.rowsPerWave = r.g.dim >= 2 ? s->rowsPerWave : 1;
+	p.wavesPerBlock = s->wavesPerBlock;
 	if (hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed");
 	s->pools.push_back(std::move(p));
 	return (int)s->pools.size() - 1;
@@ -370,6 +375,8 @@ extern "C" int hydro_server_set_launch(hydro_server *s, int rows_per_wave, int w
 	if (!s) return fail("NULL handle");
 	if (waves_per_block != 0 && waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
 		return fail("waves_per_block must be 1, 2 or 4");
+	if (rows_per_wave > 0) s->rowsPerWave = rows_per_wave;
+	if (waves_per_block > 0) s->wavesPerBlock = waves_per_block;
 	for (auto &p : s->pools) {
 		if (rows_per_wave > 0 && p.cfg.dim >= 2) p.rowsPerWave = rows_per_wave;
 		if (waves_per_block > 0) p.wavesPerBlock = waves_per_block;

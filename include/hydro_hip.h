@@ -285,7 +285,9 @@ int hydro_server_poll(hydro_server *s, long long *ids, int max, int *n);
  * 1 non-finite values in the state. Releases the request. */
 int hydro_server_result(hydro_server *s, long long id, void *U, size_t bytes, double *t, int *status);
 int hydro_server_info_get(hydro_server *s, hydro_server_info *info);
-/* launch shape of every pool (no counterpart in the reference) */
+/* launch shape of the pools, present and future (default: 12 rows per wave
+ * for 2D pools, one wave per block); 0 keeps a value. No counterpart in the
+ * reference. */
 int hydro_server_set_launch(hydro_server *s, int rows_per_wave, int waves_per_block);
 
 #ifdef __cplusplus

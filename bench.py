@@ -260,11 +260,18 @@ def serving_leg(count, seed=0, steps_per_tick=20, budget=32 << 30):
     lat, mk, _ = serve(reqs_p, budget, steps_per_tick)
     lat_s, mk_s, _ = serve_sequential(reqs, closed_loop=True)
     kinds = {k: sum(r.kind == k for r in reqs) for k in ('euler1d', 'euler2d', 'glm2d')}
+    by_kind = {}
+    for k in kinds:
+        idx = [i for i, r in enumerate(reqs_p) if r.kind == k]
+        if idx:
+            by_kind[k] = {'p50_latency_s': float(np.percentile(lat[idx], 50)),
+                          'p99_latency_s': float(np.percentile(lat[idx], 99)),
+                          'cell_updates': int(sum(reqs_p[i].cell_updates() for i in idx))}
     return {
         'workload': f'{count} solve requests (seed {seed}): 1D Euler N in [256, 65536], 2D Euler side '
                     f'log-uniform [64, 1024], 2D GLM-Maxwell side log-uniform [128, 512] (sigma 0, eps = mu = 1); '
                     f'100-2000 steps each, FP32, Roe + plm cons/minmod, RK2-TVD',
-        'mix': kinds, 'cell_updates': cu,
+        'mix': kinds, 'cell_updates': cu, 'poisson_by_kind': by_kind,
         'closed_loop_solves_per_s': capacity, 'closed_loop_gcell_updates_per_s': cu / mk_c / 1e9,
         'poisson_rate_per_s': rate, 'solves_per_s': count / mk,
         'p50_latency_s': float(np.percentile(lat, 50)), 'p99_latency_s': float(np.percentile(lat, 99)),

@@ -179,8 +179,8 @@ static int validate(const hydro_cfg *c) {
 // 2-3 resident waves per SIMD (VGPR-bound): pick the rows that minimise the
 // busiest SIMD's rows, ceil(waves / SIMDs) * (rows + warm-up), among launches
 // of at least 1.5 waves per SIMD (one lone wave cannot hide its own memory
-// latency). FP32 holds
-// twice as many waves, which hide latency better with more, shorter strips.
+// latency). FP32 holds twice as many waves, which hide latency better with
+// more, shorter strips.
 // 3D: planes per block chunk. Blocks are small (4 waves) and many resident
 // per CU, so the tail matters less than in 2D: keep at least 4096 blocks and,
 // among those chunk sizes, the fewest plane iterations per column (chunks that

@@ -84,6 +84,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	constexpr int NV = C::NV;
 	static_assert(!E::SOURCE, "3D is built for the Euler equations");
 	static_assert(E::NW == NV, "dW shares the LDS slots of the states");
+	static_assert(W >= 2, "the edge waves of a block are distinct");
 	const typename E::Par par = E::par(a);
 	const int lim = PLM ? 0 : a.limiter;
 	const int slim = a.limiter;
@@ -134,6 +135,31 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	if constexpr (PLM) { loadPlane(z0 - 2, W0); loadPlane(z0 - 1, W1); }
 	else if constexpr (LIM) { loadPlane(z0 - 2, W1); }
 	else { loadPlane(z0 - 1, W0); }
+	// loads run one plane ahead of their use, so that their latency overlaps a
+	// plane of arithmetic: the window's next plane (Wn), the edge waves' two
+	// halo rows (H0, H1), and the combine's first alpha term not taken from
+	// this stage's source (pre)
+	constexpr int AHEAD = PLM || LIM ? 2 : 1;     // iteration r consumes plane r + AHEAD
+	C Wn, H0, H1;
+	loadPlane(z0 - WARM + AHEAD, Wn);
+	auto loadHalo = [&](int zz) {
+		if (w == 0) { loadAt(y0 - 2, zz, H0); loadAt(y0 - 1, zz, H1); }
+		if (w == W - 1) { loadAt(y0 + W, zz, H0); loadAt(y0 + W + 1, zz, H1); }
+	};
+	loadHalo(z0);
+	int accPre = -1;
+	for (int k = 0; k < a.nAlpha; ++k)
+		if (accPre < 0 && a.accU[k] != a.src) accPre = k;
+	const R *Upre = accPre >= 0 ? g.U[a.accU[accPre]] : src;
+	auto loadPre = [&](int zz, R *v) {
+		long off = ((long)zz * ny + y) * g.step1 + x;
+#pragma unroll
+		for (int j = 0; j < Q; ++j) v[j] = E::active(j) ? Upre[j * g.ps + off] : R(0);
+	};
+	R pre[Q];
+#pragma unroll
+	for (int j = 0; j < Q; ++j) pre[j] = R(0);
+	if (accPre >= 0 && outLane) loadPre(z0, pre);
 
 	for (int r = z0 - WARM; r < z1; ++r) {
 		// ---- z direction: interface r+1/2, dzterm of plane r
@@ -143,7 +169,8 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			// warm-up interfaces below z0-1/2 are never used
 			const bool wantFlux = r >= z0 - 1;
 			if constexpr (PLM) {
-				loadPlane(r + 2, W2);
+				W2 = Wn;
+				if (r + 1 < z1) loadPlane(r + 3, Wn);
 				C L1, R1;
 				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
 				if (wantFlux) {
@@ -156,7 +183,8 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				}
 				Rz0 = R1;
 			} else if constexpr (LIM) {
-				loadPlane(r + 2, W2);
+				W2 = Wn;
+				if (r + 1 < z1) loadPlane(r + 3, Wn);
 				Iface<R, E> I2;
 				iface_prep<R, E, 2>(par, W1, W2, I2);
 				if (wantFlux) iface_flux<R, E, 2, true, FLIM>(par, I1, dW0, I2.dW, dt_dx2, lim, F);
@@ -168,7 +196,8 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				for (int j = 0; j < E::NW; ++j) dW0[j] = I1.dW[j];
 				I1 = I2;
 			} else {
-				loadPlane(r + 1, W1);
+				W1 = Wn;
+				if (r + 1 < z1) loadPlane(r + 2, Wn);
 				Iface<R, E> I;
 				iface_prep<R, E, 2>(par, W0, W1, I);
 				iface_flux<R, E, 2, false, 0>(par, I, nullptr, nullptr, dt_dx2, lim, F);
@@ -187,19 +216,14 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			// ---- y direction through LDS
 			lds_put<R, E, NV>(sh.u[w + 2], U.v, lane);
 			if (w == 0) {
-				C h;
-				loadAt(y0 - 2, r, h);
-				lds_put<R, E, NV>(sh.u[0], h.v, lane);
-				loadAt(y0 - 1, r, h);
-				lds_put<R, E, NV>(sh.u[1], h.v, lane);
+				lds_put<R, E, NV>(sh.u[0], H0.v, lane);
+				lds_put<R, E, NV>(sh.u[1], H1.v, lane);
 			}
 			if (w == W - 1) {
-				C h;
-				loadAt(y0 + W, r, h);
-				lds_put<R, E, NV>(sh.u[W + 2], h.v, lane);
-				loadAt(y0 + W + 1, r, h);
-				lds_put<R, E, NV>(sh.u[W + 3], h.v, lane);
+				lds_put<R, E, NV>(sh.u[W + 2], H0.v, lane);
+				lds_put<R, E, NV>(sh.u[W + 3], H1.v, lane);
 			}
+			if (r + 1 < z1) loadHalo(r + 1);
 			__syncthreads();
 
 			R Fy[Q], FyTop[Q];
@@ -345,6 +369,10 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				double acc[Q];
 #pragma unroll
 				for (int j = 0; j < Q; ++j) acc[j] = 0.;
+				R preNow[Q];
+#pragma unroll
+				for (int j = 0; j < Q; ++j) preNow[j] = pre[j];
+				if (accPre >= 0 && r + 1 < z1) loadPre(r + 1, pre);
 				for (int k = 0; k < a.nAlpha; ++k) {
 					const bool fromSrc = a.accU[k] == a.src;
 					const R *Uk = g.U[a.accU[k]];
@@ -352,7 +380,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 #pragma unroll
 					for (int j = 0; j < Q; ++j) {
 						if (!E::active(j)) continue;
-						R u = fromSrc ? U.v[j] : Uk[j * g.ps + off];
+						R u = fromSrc ? U.v[j] : (k == accPre ? preNow[j] : Uk[j * g.ps + off]);
 						acc[j] = acc[j] + (double)u * al;
 					}
 				}

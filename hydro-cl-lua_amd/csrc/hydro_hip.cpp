@@ -35,6 +35,18 @@ extern "C" int hydro_eqn_num_states(int eqn) { return num_states(eqn); }
 extern "C" int hydro_eqn_num_int_states(int eqn) { return num_int_states(eqn); }
 extern "C" void hydro_cfg_default(hydro_cfg *c) { cfg_default(c); }
 
+extern "C" int hydro_integrator_table(int integrator, int *stages, double *alpha, double *beta) {
+	RKTable t;
+	if (!rk_table(integrator, t)) return fail("integrator %d not supported", integrator);
+	if (stages) *stages = t.order;
+	for (int i = 0; i < 4; ++i)
+		for (int k = 0; k < 4; ++k) {
+			if (alpha) alpha[4 * i + k] = t.a[i][k];
+			if (beta) beta[4 * i + k] = t.b[i][k];
+		}
+	return 0;
+}
+
 extern "C" int hydro_device_count(void) {
 	int n = 0;
 	if (hipGetDeviceCount(&n) != hipSuccess) return 0;

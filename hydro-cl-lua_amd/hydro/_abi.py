@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(LIB_DIR, 'libhydro_hip.so')
 # every symbol include/hydro_hip.h declares
 EXPORTS = [
     'hydro_cfg_default', 'hydro_last_error', 'hydro_eqn_num_states', 'hydro_eqn_num_int_states',
+    'hydro_integrator_table',
     'hydro_device_count', 'hydro_create', 'hydro_destroy', 'hydro_num_cells', 'hydro_upload_U',
     'hydro_download_U', 'hydro_constrain_U', 'hydro_boundary', 'hydro_calc_dt', 'hydro_step',
     'hydro_update', 'hydro_get_time', 'hydro_set_time', 'hydro_conserved_totals', 'hydro_sync',
@@ -62,6 +63,8 @@ def lib():
     L.hydro_last_error.restype = ctypes.c_char_p
     L.hydro_eqn_num_states.argtypes = [ip]
     L.hydro_eqn_num_int_states.argtypes = [ip]
+    L.hydro_integrator_table.argtypes = [ip, ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]
     L.hydro_device_count.restype = ip
     L.hydro_create.argtypes = [cfgp, ctypes.POINTER(vp)]
     L.hydro_destroy.argtypes = [vp]

@@ -13,6 +13,9 @@ import ctypes
 import numpy as np
 
 from .. import _abi
+from .. import eqn as eqn_plugins
+from .. import flux as flux_plugins
+from .. import int as int_plugins
 from .._cfg import make_cfg, cfg_ghost_shape, INTEGRATOR_STAGES
 from ..init import euler as init_euler
 
@@ -50,10 +53,15 @@ class HipFVSolver:
                  fluxLimiter='donor cell', usePLM=None, slopeLimiter='minmod', cfl=.5,
                  fixedDT=None, mins=None, maxs=None, boundary='freeflow', boundaryFixed=None,
                  initCond=None, initCondArgs=None, real='double', eqnArgs=None, fastMath=False):
-        eqnArgs = dict(eqnArgs or {})
-        self.cfg = make_cfg(dim=dim, size=gridSize, mins=mins, maxs=maxs, eqn=eqn, flux=flux,
-                            flux_limiter=fluxLimiter, plm=usePLM, slope_limiter=slopeLimiter,
-                            integrator=integrator, cfl=cfl, fixed_dt=fixedDT, boundary=boundary,
+        # plugins by name or as objects (hydro.eqn / hydro.flux / hydro.int)
+        self.eqn = eqn_plugins.get(eqn)
+        eqnArgs = dict(getattr(self.eqn, 'guiVars', {}), **(eqnArgs or {}))
+        self.eqn.guiVars = eqnArgs
+        self.flux = flux_plugins.get(flux)
+        self.integrator = int_plugins.get(integrator)
+        self.cfg = make_cfg(dim=dim, size=gridSize, mins=mins, maxs=maxs, eqn=self.eqn.name.replace('-', '_'),
+                            flux=self.flux.name, flux_limiter=fluxLimiter, plm=usePLM, slope_limiter=slopeLimiter,
+                            integrator=self.integrator.name, cfl=cfl, fixed_dt=fixedDT, boundary=boundary,
                             boundary_fixed=boundaryFixed, precision=real,
                             gamma=eqnArgs.get('heatCapacityRatio', 7 / 5),
                             rho_min=eqnArgs.get('rhoMin', 1e-7), P_min=eqnArgs.get('PMin', 1e-7),

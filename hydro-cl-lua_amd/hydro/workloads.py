@@ -99,8 +99,11 @@ def serving_mix(count, seed=0, rate=None, real='float', fast=True):
     """requests of the serving mix, a third of each kind: 1D Euler with N in
     [256, 65536], 2D Euler with side log-uniform in [64, 1024], 2D GLM-Maxwell
     with side log-uniform in [128, 512]; steps uniform in [100, 2000]; arrival
-    times from a Poisson process of the given rate (solves/s) when rate is set"""
+    times from a Poisson process of the given rate (solves/s) when rate is set,
+    drawn from their own generator so that the requests are the same at any
+    rate"""
     rng = np.random.default_rng(seed)
+    arrivals = np.random.default_rng(seed + 1)     # the mix does not depend on the rate
     out = []
     t = 0.
     for i in range(count):
@@ -115,6 +118,6 @@ def serving_mix(count, seed=0, rate=None, real='float', fast=True):
             s = int(round(np.exp(rng.uniform(np.log(128), np.log(512)))))
             kw = glm_2d_kw(s, real, fast)
         if rate:
-            t += rng.exponential(1. / rate)
+            t += arrivals.exponential(1. / rate)
         out.append(Request(kind, kw, steps, int(rng.integers(0, 2 ** 31)), t))
     return out

@@ -132,6 +132,12 @@ const char *hydro_last_error(void);
 int hydro_eqn_num_states(int eqn);
 int hydro_eqn_num_int_states(int eqn);
 
+/* The Shu-Osher table of an integrator, as the reference's `alphas` / `betas`
+ * (hydro/int/all.lua:10-160, read by rk.lua:47-167): *stages rows, alpha and
+ * beta row-major 4 x 4 (row i: the stage's weights of U^(k), k <= i). Returns
+ * -1 for an unknown integrator. Host only. */
+int hydro_integrator_table(int integrator, int *stages, double *alpha, double *beta);
+
 /* Number of device GPUs visible (0 when none). No other call needs one. */
 int hydro_device_count(void);
 

@@ -182,7 +182,7 @@ static int validate(const hydro_cfg *c) {
 // latency). FP32 holds twice as many waves, which hide latency better with
 // more, shorter strips.
 // 3D: planes per block chunk. Blocks are small (4 waves) and many resident
-// per CU, so the tail matters less than in 2D: keep at least 4096 blocks and,
+// per CU, so the tail matters less than in 2D: keep at least 5000 blocks and,
 // among those chunk sizes, the fewest plane iterations per column (chunks that
 // divide the depth evenly, each with its 1-3 plane warm-up).
 static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
@@ -196,7 +196,7 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 		long best = 8, bestCost = -1;
 		for (long r = 8; r <= 64; ++r) {
 			long chunks = (nz + r - 1) / r;
-			if (r > 8 && perPlane * chunks < 4096) continue;
+			if (r > 8 && perPlane * chunks < 5000) continue;
 			long cost = chunks * (r + warm);
 			if (bestCost < 0 || cost < bestCost) { best = r; bestCost = cost; }
 		}

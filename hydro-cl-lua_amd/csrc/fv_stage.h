@@ -219,13 +219,23 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 		if constexpr (PLM) { loadRow(y0 - 2, W0); loadRow(y0 - 1, W1); }
 		else if constexpr (LIM) { loadRow(y0 - 2, W1); }
 		else { loadRow(y0 - 1, W0); }
-		loadRow(y0 - WARM + AHEAD, Wn);
+		if constexpr (Prefetch<R>::rows) loadRow(y0 - WARM + AHEAD, Wn);
 	}
+	// row r + AHEAD into dst at iteration r
+	auto takeRow = [&](int r, C &dst) {
+		if constexpr (Prefetch<R>::rows) {
+			dst = Wn;
+			if (r + 1 < y1) loadRow(r + AHEAD + 1, Wn);
+		} else {
+			loadRow(r + AHEAD, dst);
+		}
+	};
 	// the first alpha term that is not this stage's source is read at the
 	// output cell by the combine: also loaded one iteration ahead
 	int accPre = -1;
-	for (int k = 0; k < a.nAlpha; ++k)
-		if (accPre < 0 && a.accU[k] != a.src) accPre = k;
+	if constexpr (Prefetch<R>::combine)
+		for (int k = 0; k < a.nAlpha; ++k)
+			if (accPre < 0 && a.accU[k] != a.src) accPre = k;
 	const R *Upre = accPre >= 0 ? g.U[a.accU[accPre]] : src;
 	auto loadPre = [&](int r, R *v) {
 		long off = (long)r * g.step1 + x;
@@ -246,8 +256,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			const bool wantFlux = !(PLM && r == y0 - WARM);
 			if constexpr (PLM) {
 				// W0 = U(r), W1 = U(r+1): cell r+1's edges, interface r+1/2
-				W2 = Wn;
-				if (r + 1 < y1) loadRow(r + 3, Wn);
+				takeRow(r, W2);
 				C L1, R1;
 				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
 				if (wantFlux) {
@@ -261,8 +270,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				Ry0 = R1;
 			} else if constexpr (LIM) {
 				// W1 = U(r+1): interface r+3/2, flux at r+1/2 from its neighbours
-				W2 = Wn;
-				if (r + 1 < y1) loadRow(r + 3, Wn);
+				takeRow(r, W2);
 				Iface<R, E> I2;
 				iface_prep<R, E, 1>(par, W1, W2, I2);
 				iface_flux<R, E, 1, true, FLIM>(par, I1, dW0, I2.dW, dt_dx1, lim, F);
@@ -271,8 +279,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				I1 = I2;
 			} else {
 				// W0 = U(r): interface r+1/2
-				W1 = Wn;
-				if (r + 1 < y1) loadRow(r + 2, Wn);
+				takeRow(r, W1);
 				Iface<R, E> I;
 				iface_prep<R, E, 1>(par, W0, W1, I);
 				iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, F);

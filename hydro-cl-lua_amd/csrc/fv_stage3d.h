@@ -141,15 +141,24 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	// this stage's source (pre)
 	constexpr int AHEAD = PLM || LIM ? 2 : 1;     // iteration r consumes plane r + AHEAD
 	C Wn, H0, H1;
-	loadPlane(z0 - WARM + AHEAD, Wn);
+	if constexpr (Prefetch<R>::rows) loadPlane(z0 - WARM + AHEAD, Wn);
+	auto takePlane = [&](int r, C &dst) {
+		if constexpr (Prefetch<R>::rows) {
+			dst = Wn;
+			if (r + 1 < z1) loadPlane(r + AHEAD + 1, Wn);
+		} else {
+			loadPlane(r + AHEAD, dst);
+		}
+	};
 	auto loadHalo = [&](int zz) {
 		if (w == 0) { loadAt(y0 - 2, zz, H0); loadAt(y0 - 1, zz, H1); }
 		if (w == W - 1) { loadAt(y0 + W, zz, H0); loadAt(y0 + W + 1, zz, H1); }
 	};
-	loadHalo(z0);
+	if constexpr (Prefetch<R>::rows) loadHalo(z0);
 	int accPre = -1;
-	for (int k = 0; k < a.nAlpha; ++k)
-		if (accPre < 0 && a.accU[k] != a.src) accPre = k;
+	if constexpr (Prefetch<R>::combine)
+		for (int k = 0; k < a.nAlpha; ++k)
+			if (accPre < 0 && a.accU[k] != a.src) accPre = k;
 	const R *Upre = accPre >= 0 ? g.U[a.accU[accPre]] : src;
 	auto loadPre = [&](int zz, R *v) {
 		long off = ((long)zz * ny + y) * g.step1 + x;
@@ -169,8 +178,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			// warm-up interfaces below z0-1/2 are never used
 			const bool wantFlux = r >= z0 - 1;
 			if constexpr (PLM) {
-				W2 = Wn;
-				if (r + 1 < z1) loadPlane(r + 3, Wn);
+				takePlane(r, W2);
 				C L1, R1;
 				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
 				if (wantFlux) {
@@ -183,8 +191,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				}
 				Rz0 = R1;
 			} else if constexpr (LIM) {
-				W2 = Wn;
-				if (r + 1 < z1) loadPlane(r + 3, Wn);
+				takePlane(r, W2);
 				Iface<R, E> I2;
 				iface_prep<R, E, 2>(par, W1, W2, I2);
 				if (wantFlux) iface_flux<R, E, 2, true, FLIM>(par, I1, dW0, I2.dW, dt_dx2, lim, F);
@@ -196,8 +203,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				for (int j = 0; j < E::NW; ++j) dW0[j] = I1.dW[j];
 				I1 = I2;
 			} else {
-				W1 = Wn;
-				if (r + 1 < z1) loadPlane(r + 2, Wn);
+				takePlane(r, W1);
 				Iface<R, E> I;
 				iface_prep<R, E, 2>(par, W0, W1, I);
 				iface_flux<R, E, 2, false, 0>(par, I, nullptr, nullptr, dt_dx2, lim, F);
@@ -215,6 +221,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 
 			// ---- y direction through LDS
 			lds_put<R, E, NV>(sh.u[w + 2], U.v, lane);
+			if constexpr (!Prefetch<R>::rows) loadHalo(r);
 			if (w == 0) {
 				lds_put<R, E, NV>(sh.u[0], H0.v, lane);
 				lds_put<R, E, NV>(sh.u[1], H1.v, lane);
@@ -223,7 +230,9 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				lds_put<R, E, NV>(sh.u[W + 2], H0.v, lane);
 				lds_put<R, E, NV>(sh.u[W + 3], H1.v, lane);
 			}
-			if (r + 1 < z1) loadHalo(r + 1);
+			if constexpr (Prefetch<R>::rows) {
+				if (r + 1 < z1) loadHalo(r + 1);
+			}
 			__syncthreads();
 
 			R Fy[Q], FyTop[Q];

@@ -174,20 +174,17 @@ static int validate(const hydro_cfg *c) {
 }
 
 // Rows per wave of the plain stage kernel. Each wave marches `rows` rows of a
-// 60-column strip after a warm-up of 1-3 rows, so short strips waste work on
-// warm-up and long ones leave SIMDs idle in the last wave round. FP64 runs at
-// 2-3 resident waves per SIMD (VGPR-bound): pick the rows that minimise the
-// busiest SIMD's rows, ceil(waves / SIMDs) * (rows + warm-up), among launches
-// of at least 1.5 waves per SIMD (one lone wave cannot hide its own memory
-// latency). FP32 holds twice as many waves, which hide latency better with
-// more, shorter strips.
+// 60-column strip after a warm-up of 1-3 rows: short strips waste work on the
+// warm-up, long ones leave too few waves to hide latency (2-3 resident waves
+// per SIMD at the kernel's registers). Measured on KH (rows 2..16 at 1024^2
+// and 2048^2, FP64 and FP32), about 8 waves per SIMD over the launch with at
+// least 4 rows is best: rows = clamp(strips * ny / 8192, 4, 64).
 // 3D: planes per block chunk. Blocks are small (4 waves) and many resident
 // per CU, so the tail matters less than in 2D: keep at least 5000 blocks and,
 // among those chunk sizes, the fewest plane iterations per column (chunks that
 // divide the depth evenly, each with its 1-3 plane warm-up).
 static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 	if (cfg.dim < 2) return 1;
-	const long simds = 1024;      // 256 CUs x 4 SIMDs
 	long stripsX = (nx + STRIP_OUT - 1) / STRIP_OUT;
 	if (cfg.dim == 3) {
 		long W = cfg.precision == HYDRO_REAL_F32 ? Block3<float>::W : Block3<double>::W;
@@ -202,19 +199,8 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 		}
 		return (int)best;
 	}
-	if (cfg.precision == HYDRO_REAL_F32) {
-		long rows = (stripsX * (long)ny + 4607) / 4608;
-		return (int)std::max(4L, std::min(64L, rows));
-	}
-	int warm = cfg.plm ? 2 : (cfg.flux_limiter ? 3 : 1);
-	long best = -1, bestCost = 0;
-	for (long r = 4; r <= 64; ++r) {
-		long waves = stripsX * ((ny + r - 1) / r);
-		if (r > 4 && 2 * waves < 3 * simds) continue;
-		long cost = (waves + simds - 1) / simds * (r + warm);
-		if (best < 0 || cost < bestCost) { best = r; bestCost = cost; }
-	}
-	return (int)best;
+	long rows = (stripsX * (long)ny + 8191) / 8192;
+	return (int)std::max(4L, std::min(64L, rows));
 }
 
 static void grid_init(Grid &g, const hydro_cfg &cfg) {

@@ -34,6 +34,13 @@ __device__ __forceinline__ void load_cell(const GridDesc<R> &g, const R *base, c
 	Cell<R, E> &c)
 {
 	long off = (long)my.idx * g.step1 + mx.idx;
+	// axis by axis, a later axis wins: the last fixed axis gives the value and
+	// the mirrors of the axes after it flip it. One table lookup from a face
+	// index chosen by value (selecting between two lookups would copy the
+	// whole argument block to scratch memory).
+	int fixedFace = -1, lastFixed = -1;
+	if (mx.fixedFace >= 0) { fixedFace = mx.fixedFace; lastFixed = 0; }
+	if (my.fixedFace >= 0) { fixedFace = my.fixedFace; lastFixed = 1; }
 #pragma unroll
 	for (int f = 0; f < Cell<R, E>::NV; ++f) {
 		if (!E::active(f)) {
@@ -43,10 +50,10 @@ __device__ __forceinline__ void load_cell(const GridDesc<R> &g, const R *base, c
 		R v;
 		if (f < E::Q) v = base[f * g.ps + off];
 		else v = g.stat[(f - E::Q) * g.ps + off];
-		if (mx.fixedFace >= 0) v = g.fixedv[mx.fixedFace][f];
-		else if (mx.mirror && E::reflects(f, 0)) v = -v;
-		if (my.fixedFace >= 0) v = g.fixedv[my.fixedFace][f];
-		else if (my.mirror && E::reflects(f, 1)) v = -v;
+		R fv = g.fixedv[fixedFace < 0 ? 0 : fixedFace][f];
+		v = fixedFace >= 0 ? fv : v;
+		if (lastFixed < 0 && mx.mirror && E::reflects(f, 0)) v = -v;
+		if (lastFixed < 1 && my.mirror && E::reflects(f, 1)) v = -v;
 		c.v[f] = v;
 	}
 }
@@ -247,7 +254,13 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	for (int j = 0; j < Q; ++j) pre[j] = R(0);
 	if (accPre >= 0 && outLane && y0 < y1) loadPre(y0, pre);
 
-	for (int r = y0 - WARM; r < y1; ++r) {
+	// two passes: the warm-up rows (y pipeline only), then the output rows, so
+	// that the output rows' body is one block with the y and x work together
+#pragma unroll
+	for (int pass = 0; pass < 2; ++pass) {
+	const int rBegin = pass == 0 ? y0 - WARM : y0;
+	const int rEnd = pass == 0 ? y0 : y1;
+	for (int r = rBegin; r < rEnd; ++r) {
 		// ---- y direction: interface r+1/2, dyterm of row r
 		R dyterm[Q];
 		if constexpr (DIM >= 2) {
@@ -292,7 +305,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			}
 		}
 
-		if (r >= y0) {
+		if (pass == 1) {
 			// ---- x direction on row r (lanes)
 			C U;
 			if constexpr (DIM >= 2) U = W0;
@@ -405,6 +418,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			if constexpr (PLM || LIM) { W0 = W1; W1 = W2; }
 			else { W0 = W1; }
 		}
+	}
 	}
 	return dtLocal;
 }

@@ -170,7 +170,13 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	for (int j = 0; j < Q; ++j) pre[j] = R(0);
 	if (accPre >= 0 && outLane) loadPre(z0, pre);
 
-	for (int r = z0 - WARM; r < z1; ++r) {
+	// two passes: the warm-up planes (z pipeline only), then the output
+	// planes, so that the output planes' body is one block with all the work
+#pragma unroll
+	for (int pass = 0; pass < 2; ++pass) {
+	const int rBegin = pass == 0 ? z0 - WARM : z0;
+	const int rEnd = pass == 0 ? z0 : z1;
+	for (int r = rBegin; r < rEnd; ++r) {
 		// ---- z direction: interface r+1/2, dzterm of plane r
 		R dzterm[Q];
 		{
@@ -216,7 +222,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			}
 		}
 
-		if (r >= z0) {
+		if (pass == 1) {
 			const C &U = W0;     // plane r of this wave's row
 
 			// ---- y direction through LDS
@@ -433,6 +439,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 		// ---- advance the plane window
 		if constexpr (PLM || LIM) { W0 = W1; W1 = W2; }
 		else { W0 = W1; }
+	}
 	}
 	return dtLocal;
 }

@@ -14,25 +14,6 @@ constexpr int NG = 2;              // numGhost (gridsolver.lua:41)
 constexpr int WAVE = 64;           // CDNA wavefront
 constexpr int STRIP_OUT = WAVE - 4;// output columns per wave (2-cell halo each side)
 
-// Loads one row / plane ahead of their use in the stage kernels: bit 0 the
-// marching window's next row (and the 3D halo rows), bit 1 the combine's U^n.
-// They trade registers for hidden memory latency. Measured at 1024^2 KH and
-// 256^3 blast: FP64 gains up to a few percent with bit 0 (its occupancy is set
-// by the arithmetic's registers); in FP32 bit 1 costs 13% and bit 0 in 3D 30%
-// (a wave per SIMD less). -DHYDRO_PREFETCH64 / -DHYDRO_PREFETCH32 override.
-#ifndef HYDRO_PREFETCH64
-#define HYDRO_PREFETCH64 1
-#endif
-#ifndef HYDRO_PREFETCH32
-#define HYDRO_PREFETCH32 0
-#endif
-template <typename R>
-struct Prefetch {
-	static constexpr int bits = sizeof(R) == 8 ? HYDRO_PREFETCH64 : HYDRO_PREFETCH32;
-	static constexpr bool rows = (bits & 1) != 0;
-	static constexpr bool combine = (bits & 2) != 0;
-};
-
 // 3D stage kernel: waves per block (rows of a strip that share one plane
 // through LDS). Four, so that two blocks fit per CU at the kernel's register
 // use and one block's barrier waits are covered by the other's work (8 waves

@@ -178,7 +178,7 @@ static int validate(const hydro_cfg *c) {
 // warm-up, long ones leave too few waves to hide latency (2-3 resident waves
 // per SIMD at the kernel's registers). Measured on KH (rows 2..16 at 1024^2
 // and 2048^2, FP64 and FP32), about 8 waves per SIMD over the launch with at
-// least 4 rows is best: rows = clamp(strips * ny / 8192, 4, 64).
+// least 5 rows is best: rows = clamp(strips * ny / 8192, 5, 64).
 // 3D: planes per block chunk. Blocks are small (4 waves) and many resident
 // per CU, so the tail matters less than in 2D: keep at least 5000 blocks and,
 // among those chunk sizes, the fewest plane iterations per column (chunks that
@@ -200,7 +200,7 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 		return (int)best;
 	}
 	long rows = (stripsX * (long)ny + 8191) / 8192;
-	return (int)std::max(4L, std::min(64L, rows));
+	return (int)std::max(5L, std::min(64L, rows));
 }
 
 static void grid_init(Grid &g, const hydro_cfg &cfg) {

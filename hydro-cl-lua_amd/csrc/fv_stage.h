@@ -218,41 +218,11 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 
 	// iterations before y0 only fill the pipeline
 	constexpr int WARM = DIM < 2 ? 0 : (PLM ? 2 : (LIM ? 3 : 1));
-	// the row an iteration brings in is loaded one iteration ahead (Wn), so
-	// its memory latency overlaps a whole iteration of arithmetic
-	constexpr int AHEAD = PLM || LIM ? 2 : 1;     // iteration r consumes row r + AHEAD
-	C Wn;
 	if constexpr (DIM >= 2) {
 		if constexpr (PLM) { loadRow(y0 - 2, W0); loadRow(y0 - 1, W1); }
 		else if constexpr (LIM) { loadRow(y0 - 2, W1); }
 		else { loadRow(y0 - 1, W0); }
-		if constexpr (Prefetch<R>::rows) loadRow(y0 - WARM + AHEAD, Wn);
 	}
-	// row r + AHEAD into dst at iteration r
-	auto takeRow = [&](int r, C &dst) {
-		if constexpr (Prefetch<R>::rows) {
-			dst = Wn;
-			if (r + 1 < y1) loadRow(r + AHEAD + 1, Wn);
-		} else {
-			loadRow(r + AHEAD, dst);
-		}
-	};
-	// the first alpha term that is not this stage's source is read at the
-	// output cell by the combine: also loaded one iteration ahead
-	int accPre = -1;
-	if constexpr (Prefetch<R>::combine)
-		for (int k = 0; k < a.nAlpha; ++k)
-			if (accPre < 0 && a.accU[k] != a.src) accPre = k;
-	const R *Upre = accPre >= 0 ? g.U[a.accU[accPre]] : src;
-	auto loadPre = [&](int r, R *v) {
-		long off = (long)r * g.step1 + x;
-#pragma unroll
-		for (int j = 0; j < Q; ++j) v[j] = E::active(j) ? Upre[j * g.ps + off] : R(0);
-	};
-	R pre[Q];
-#pragma unroll
-	for (int j = 0; j < Q; ++j) pre[j] = R(0);
-	if (accPre >= 0 && outLane && y0 < y1) loadPre(y0, pre);
 
 	// two passes: the warm-up rows (y pipeline only), then the output rows, so
 	// that the output rows' body is one block with the y and x work together
@@ -269,7 +239,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			const bool wantFlux = !(PLM && r == y0 - WARM);
 			if constexpr (PLM) {
 				// W0 = U(r), W1 = U(r+1): cell r+1's edges, interface r+1/2
-				takeRow(r, W2);
+				loadRow(r + 2, W2);
 				C L1, R1;
 				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
 				if (wantFlux) {
@@ -283,7 +253,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				Ry0 = R1;
 			} else if constexpr (LIM) {
 				// W1 = U(r+1): interface r+3/2, flux at r+1/2 from its neighbours
-				takeRow(r, W2);
+				loadRow(r + 2, W2);
 				Iface<R, E> I2;
 				iface_prep<R, E, 1>(par, W1, W2, I2);
 				iface_flux<R, E, 1, true, FLIM>(par, I1, dW0, I2.dW, dt_dx1, lim, F);
@@ -292,7 +262,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				I1 = I2;
 			} else {
 				// W0 = U(r): interface r+1/2
-				takeRow(r, W1);
+				loadRow(r + 1, W1);
 				Iface<R, E> I;
 				iface_prep<R, E, 1>(par, W0, W1, I);
 				iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, F);
@@ -360,10 +330,6 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				double acc[Q];
 #pragma unroll
 				for (int j = 0; j < Q; ++j) acc[j] = 0.;
-				R preNow[Q];
-#pragma unroll
-				for (int j = 0; j < Q; ++j) preNow[j] = pre[j];
-				if (accPre >= 0 && r + 1 < y1) loadPre(r + 1, pre);
 				for (int k = 0; k < a.nAlpha; ++k) {
 					const bool fromSrc = a.accU[k] == a.src;
 					const R *Uk = g.U[a.accU[k]];
@@ -371,7 +337,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 #pragma unroll
 					for (int j = 0; j < Q; ++j) {
 						if (!E::active(j)) continue;
-						R u = fromSrc ? U.v[j] : (k == accPre ? preNow[j] : Uk[j * g.ps + off]);
+						R u = fromSrc ? U.v[j] : Uk[j * g.ps + off];
 						acc[j] = acc[j] + (double)u * al;
 					}
 				}
@@ -452,16 +418,8 @@ __device__ __forceinline__ void stage_reduce(const StageArgs<R> &a, const GridDe
 
 // ---------------------------------------------------------- the kernels
 // Plain form: one grid, one wave per strip.
-// Occupancy hint of the plain kernel (compile-time tuning; none by default):
-// -DHYDRO_STAGE_WPE=n asks for at least n waves per SIMD.
-#ifdef HYDRO_STAGE_WPE
-#define HYDRO_STAGE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(HYDRO_STAGE_WPE)))
-#else
-#define HYDRO_STAGE_OCCUPANCY
-#endif
-
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
-__global__ void __launch_bounds__(256) HYDRO_STAGE_OCCUPANCY stage_kernel(const StageArgs<R> a)
+__global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 {
 	const int lane = threadIdx.x & 63;
 	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;

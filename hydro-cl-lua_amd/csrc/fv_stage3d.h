@@ -135,40 +135,11 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	if constexpr (PLM) { loadPlane(z0 - 2, W0); loadPlane(z0 - 1, W1); }
 	else if constexpr (LIM) { loadPlane(z0 - 2, W1); }
 	else { loadPlane(z0 - 1, W0); }
-	// loads run one plane ahead of their use, so that their latency overlaps a
-	// plane of arithmetic: the window's next plane (Wn), the edge waves' two
-	// halo rows (H0, H1), and the combine's first alpha term not taken from
-	// this stage's source (pre)
-	constexpr int AHEAD = PLM || LIM ? 2 : 1;     // iteration r consumes plane r + AHEAD
-	C Wn, H0, H1;
-	if constexpr (Prefetch<R>::rows) loadPlane(z0 - WARM + AHEAD, Wn);
-	auto takePlane = [&](int r, C &dst) {
-		if constexpr (Prefetch<R>::rows) {
-			dst = Wn;
-			if (r + 1 < z1) loadPlane(r + AHEAD + 1, Wn);
-		} else {
-			loadPlane(r + AHEAD, dst);
-		}
-	};
+	C H0, H1;
 	auto loadHalo = [&](int zz) {
 		if (w == 0) { loadAt(y0 - 2, zz, H0); loadAt(y0 - 1, zz, H1); }
 		if (w == W - 1) { loadAt(y0 + W, zz, H0); loadAt(y0 + W + 1, zz, H1); }
 	};
-	if constexpr (Prefetch<R>::rows) loadHalo(z0);
-	int accPre = -1;
-	if constexpr (Prefetch<R>::combine)
-		for (int k = 0; k < a.nAlpha; ++k)
-			if (accPre < 0 && a.accU[k] != a.src) accPre = k;
-	const R *Upre = accPre >= 0 ? g.U[a.accU[accPre]] : src;
-	auto loadPre = [&](int zz, R *v) {
-		long off = ((long)zz * ny + y) * g.step1 + x;
-#pragma unroll
-		for (int j = 0; j < Q; ++j) v[j] = E::active(j) ? Upre[j * g.ps + off] : R(0);
-	};
-	R pre[Q];
-#pragma unroll
-	for (int j = 0; j < Q; ++j) pre[j] = R(0);
-	if (accPre >= 0 && outLane) loadPre(z0, pre);
 
 	// two passes: the warm-up planes (z pipeline only), then the output
 	// planes, so that the output planes' body is one block with all the work
@@ -184,7 +155,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			// warm-up interfaces below z0-1/2 are never used
 			const bool wantFlux = r >= z0 - 1;
 			if constexpr (PLM) {
-				takePlane(r, W2);
+				loadPlane(r + 2, W2);
 				C L1, R1;
 				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
 				if (wantFlux) {
@@ -197,7 +168,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				}
 				Rz0 = R1;
 			} else if constexpr (LIM) {
-				takePlane(r, W2);
+				loadPlane(r + 2, W2);
 				Iface<R, E> I2;
 				iface_prep<R, E, 2>(par, W1, W2, I2);
 				if (wantFlux) iface_flux<R, E, 2, true, FLIM>(par, I1, dW0, I2.dW, dt_dx2, lim, F);
@@ -209,7 +180,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				for (int j = 0; j < E::NW; ++j) dW0[j] = I1.dW[j];
 				I1 = I2;
 			} else {
-				takePlane(r, W1);
+				loadPlane(r + 1, W1);
 				Iface<R, E> I;
 				iface_prep<R, E, 2>(par, W0, W1, I);
 				iface_flux<R, E, 2, false, 0>(par, I, nullptr, nullptr, dt_dx2, lim, F);
@@ -227,7 +198,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 
 			// ---- y direction through LDS
 			lds_put<R, E, NV>(sh.u[w + 2], U.v, lane);
-			if constexpr (!Prefetch<R>::rows) loadHalo(r);
+			loadHalo(r);
 			if (w == 0) {
 				lds_put<R, E, NV>(sh.u[0], H0.v, lane);
 				lds_put<R, E, NV>(sh.u[1], H1.v, lane);
@@ -235,9 +206,6 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			if (w == W - 1) {
 				lds_put<R, E, NV>(sh.u[W + 2], H0.v, lane);
 				lds_put<R, E, NV>(sh.u[W + 3], H1.v, lane);
-			}
-			if constexpr (Prefetch<R>::rows) {
-				if (r + 1 < z1) loadHalo(r + 1);
 			}
 			__syncthreads();
 
@@ -384,10 +352,6 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				double acc[Q];
 #pragma unroll
 				for (int j = 0; j < Q; ++j) acc[j] = 0.;
-				R preNow[Q];
-#pragma unroll
-				for (int j = 0; j < Q; ++j) preNow[j] = pre[j];
-				if (accPre >= 0 && r + 1 < z1) loadPre(r + 1, pre);
 				for (int k = 0; k < a.nAlpha; ++k) {
 					const bool fromSrc = a.accU[k] == a.src;
 					const R *Uk = g.U[a.accU[k]];
@@ -395,7 +359,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 #pragma unroll
 					for (int j = 0; j < Q; ++j) {
 						if (!E::active(j)) continue;
-						R u = fromSrc ? U.v[j] : (k == accPre ? preNow[j] : Uk[j * g.ps + off]);
+						R u = fromSrc ? U.v[j] : Uk[j * g.ps + off];
 						acc[j] = acc[j] + (double)u * al;
 					}
 				}

@@ -66,7 +66,11 @@ __host__ __device__ constexpr bool limited_symmetric(int LIM) {
 template <int LIM, bool FAST, typename R>
 __device__ __forceinline__ R limited(int index, R num, R den) {
 	using M = Math<R, FAST>;
-	if constexpr (FAST && limited_symmetric(LIM)) {
+	if constexpr (FAST && LIM == HYDRO_LIMITER_MINMOD) {
+		// minmod(a, b): the operand nearer 0 when both have one sign, else 0,
+		// without compares: max(min(a, b), 0) + min(max(a, b), 0)
+		return fmax(fmin(num, den), R(0)) + fmin(fmax(num, den), R(0));
+	} else if constexpr (FAST && limited_symmetric(LIM)) {
 		const bool same = (num > R(0) && den > R(0)) || (num < R(0) && den < R(0));
 		const R a = fabs(num), b = fabs(den);
 		R m;

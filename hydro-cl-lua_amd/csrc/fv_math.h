@@ -40,11 +40,9 @@ struct Math<double, true> {
 		e = __builtin_fma(-y, r, 1.);
 		return __builtin_fma(e, r, r);
 	}
-	__device__ static __forceinline__ double div_inv(double x, double y, double inv) {
-		double q = x * inv;
-		double e = __builtin_fma(-y, q, x);
-		return __builtin_fma(e, inv, q);
-	}
+	// x / y given inv = rcp(y): x * inv, within 1.5 ulp of the quotient (inv
+	// is faithful; the residual correction bought half an ulp for two FMAs)
+	__device__ static __forceinline__ double div_inv(double x, double, double inv) { return x * inv; }
 	// positive, normal divisors
 	__device__ static __forceinline__ double div_pos(double x, double y) {
 		double r = __builtin_amdgcn_rcp(y);

@@ -12,7 +12,16 @@ namespace hydro {
 
 constexpr int NG = 2;              // numGhost (gridsolver.lua:41)
 constexpr int WAVE = 64;           // CDNA wavefront
-constexpr int STRIP_OUT = WAVE - 4;// output columns per wave (2-cell halo each side)
+constexpr int STRIP_OUT = WAVE;    // output columns per wave strip with edge passes
+// Strips whose edge interfaces come from an edge pass output 64 columns; the
+// others load a 2-cell halo in lanes 0-1 and 62-63 and output 60: 1D strips
+// (a single row, too short to repay the pass) and GLM-Maxwell (12 state reals
+// per cell make the pass's gathers cost more than the halo lanes; measured).
+__host__ __device__ constexpr int strip_out(bool edges) { return edges ? STRIP_OUT : WAVE - 2 * NG; }
+__host__ __device__ constexpr bool strip_edges(int dim, bool glm) { return dim >= 2 && !glm; }
+constexpr int EDGE_ROWS = WAVE / 2;// rows per wave item at most (2D): the strip-edge
+                                   // interfaces of each row take a lane per side
+constexpr int EDGE_PLANES = 16;    // planes per block chunk at most (3D)
 
 // 3D stage kernel: waves per block (rows of a strip that share one plane
 // through LDS). Four, so that two blocks fit per CU at the kernel's register
@@ -105,7 +114,9 @@ struct StageArgs {
 };
 
 // ---------------------------------------------------------------- lane moves
-// Lane l receives lane l-1's value (wave_shr:1) / lane l+1's (wave_shl:1).
+// Lane l receives lane l-1's value (wave_shr:1) / lane l+1's (wave_shl:1);
+// the lane without a source (0 / 63) receives 0. With zero fill, FP32 users
+// of a shifted value take the shift as a DPP operand (no move).
 __device__ __forceinline__ int dpp_from_left_i(int v) {
 	return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);
 }

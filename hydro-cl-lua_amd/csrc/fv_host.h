@@ -173,25 +173,40 @@ static int validate(const hydro_cfg *c) {
 	return 0;
 }
 
-// Rows per wave of the plain stage kernel. Each wave marches `rows` rows of a
-// 60-column strip after a warm-up of 1-3 rows: short strips waste work on the
-// warm-up, long ones leave too few waves to hide latency (2-3 resident waves
-// per SIMD at the kernel's registers). Measured on KH (rows 2..16 at 1024^2
-// and 2048^2, FP64 and FP32), about 8 waves per SIMD over the launch with at
-// least 5 rows is best: rows = clamp(strips * ny / 8192, 5, 64).
-// 3D: planes per block chunk. Blocks are small (4 waves) and many resident
+// Compute units of the current device (256 on MI355X; also when none is found).
+static long device_cus() {
+	int dev = 0, cus = 0;
+	if (hipGetDevice(&dev) != hipSuccess ||
+		hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+		return 256;
+	return cus;
+}
+
+// Rows per wave of the plain stage kernel (2D), from KH sweeps (rows 3..32 at
+// 512^2 .. 4096^2, fast math). Each wave marches `rows` rows of a 64-column
+// strip after a warm-up of 1-3 rows and its strip-edge pass (about 1.5 rows of
+// work). FP64 is VALU-bound at 3 resident waves per SIMD: blocks of 4 waves
+// are dealt to the CUs, a CU runs L = ceil(blocks / CUs) of them, and the
+// launch takes about L * (rows + warm + 1.5) / eff(L), eff(L) (fitted) being
+// how well L waves per SIMD hide latency; the rows in 4..32 with the least
+// time are taken (32 being the most rows whose strip edges one wave computes
+// in a single pass). FP32 is latency-bound and gains from many short waves:
+// rows = clamp(strips * ny / 4096, 4, 8) (4 at 512^2 .. 1024^2, 8 from 2048^2;
+// longer waves leave a costly partial last round).
+// 3D: planes per block chunk (at most 16). Blocks are small (4 waves) and many resident
 // per CU, so the tail matters less than in 2D: keep at least 5000 blocks and,
 // among those chunk sizes, the fewest plane iterations per column (chunks that
 // divide the depth evenly, each with its 1-3 plane warm-up).
 static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 	if (cfg.dim < 2) return 1;
-	long stripsX = (nx + STRIP_OUT - 1) / STRIP_OUT;
+	const int so = strip_out(strip_edges(cfg.dim, cfg.eqn == HYDRO_EQN_GLM_MAXWELL));
+	long stripsX = (nx + so - 1) / so;
+	int warm = cfg.plm ? 2 : (cfg.flux_limiter ? 3 : 1);
 	if (cfg.dim == 3) {
 		long W = cfg.precision == HYDRO_REAL_F32 ? Block3<float>::W : Block3<double>::W;
 		long perPlane = stripsX * ((ny + W - 1) / W);
-		int warm = cfg.plm ? 2 : (cfg.flux_limiter ? 3 : 1);
 		long best = 8, bestCost = -1;
-		for (long r = 8; r <= 64; ++r) {
+		for (long r = 8; r <= EDGE_PLANES; ++r) {
 			long chunks = (nz + r - 1) / r;
 			if (r > 8 && perPlane * chunks < 5000) continue;
 			long cost = chunks * (r + warm);
@@ -199,8 +214,19 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 		}
 		return (int)best;
 	}
-	long rows = (stripsX * (long)ny + 8191) / 8192;
-	return (int)std::max(5L, std::min(64L, rows));
+	if (cfg.precision == HYDRO_REAL_F32)
+		return (int)std::max(4L, std::min(8L, (stripsX * (long)ny + 4095) / 4096));
+	static const double eff[4] = {0, .7, .9, 1.};
+	const long cus = device_cus();
+	int best = 4;
+	double bestT = -1;
+	for (int r = 4; r <= EDGE_ROWS; ++r) {
+		long blocks = (stripsX * ((ny + r - 1) / r) + 3) / 4;
+		long L = (blocks + cus - 1) / cus;
+		double t = (double)L * (r + warm + 1.5) / eff[std::min(L, 3L)];
+		if (bestT < 0 || t < bestT * (1 - 1e-9)) { best = r; bestT = t; }
+	}
+	return best;
 }
 
 static void grid_init(Grid &g, const hydro_cfg &cfg) {
@@ -316,7 +342,8 @@ static void grid_desc(const Grid &g, GridDesc<R> &d) {
 	}
 	d.cfl = c.cfl;
 	d.rowsPerWave = g.rowsPerWave;
-	d.stripsX = (g.n[0] + STRIP_OUT - 1) / STRIP_OUT;
+	const int so = strip_out(strip_edges(g.dim, g.cfg.eqn == HYDRO_EQN_GLM_MAXWELL));
+	d.stripsX = (g.n[0] + so - 1) / so;
 	if (g.dim == 3) {
 		d.stripsY = (g.n[1] + Block3<R>::W - 1) / Block3<R>::W;
 		d.stripsZ = (g.n[2] + g.rowsPerWave - 1) / g.rowsPerWave;

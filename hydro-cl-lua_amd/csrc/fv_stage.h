@@ -6,13 +6,17 @@
 // on the last stage also the per-cell CFL dt of the new state and its
 // min-reduction (calcDT.cl:38-73, solverbase.lua:3004-3023).
 //
-// Work decomposition: one 64-lane wave owns a strip of 64 columns (60 output
-// columns plus a 2-cell halo each side) and marches down the strip in y.
-// x neighbours are exchanged with DPP wave shifts; y neighbours stay in
-// registers (a sliding window of rows), so every state value is loaded once
-// per stage from HBM/L2 and no LDS or barriers are needed. Ghost cells are not
-// stored: loads outside the interior are mapped through the boundary
-// conditions.
+// Work decomposition: one 64-lane wave owns a strip of 64 columns, one output
+// column per lane, and marches down the strip in y. x neighbours are exchanged
+// with DPP wave shifts; y neighbours stay in registers (a sliding window of
+// rows), so every state value is loaded once per stage from HBM/L2. The two
+// interfaces at the strip's edges need cells outside the strip: before its
+// rows, the wave computes them for all its rows at once (a lane per row and
+// side, at most 32 rows) and keeps them in its own LDS slice, from which lanes
+// 0 and 63 take them row by row. A 1D strip has one row, too few to repay that
+// pass: there lanes 0-1 and 62-63 load a 2-cell halo and 60 lanes output.
+// Ghost cells are not stored: loads outside the interior are mapped through
+// the boundary conditions.
 #pragma once
 
 #include "fv_common.h"
@@ -165,13 +169,181 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 	for (int j = 0; j < E::Q; ++j) F[j] = Y[j] + I.Favg[j];
 }
 
+// ------------------------------------------------------------ strip edges
+// One wave's LDS slice for up to NR rows: for row j of the wave, slot j
+// describes the strip's left edge interface and slot NR + j its right edge
+// interface: the area-weighted x flux, the neighbour cell outside the strip
+// (needed by PLM slopes and by a source term reading x neighbours), and the
+// interface's characteristic jump dW (needed by the flux limiter).
+template <typename R, class E, bool PLM, bool LIM, int NR>
+struct XEdge {
+	static constexpr bool U = PLM || E::SOURCE;
+	static constexpr int S = 2 * NR;
+	R fa[E::Q][S];
+	R u[U ? Cell<R, E>::NV : 1][S];
+	R dw[LIM ? E::NW : 1][S];
+};
+
+// x flux (area-weighted) and dW of the interface xb - 1/2 from the cells
+// xb-2 .. xb+1 of one row: the same arithmetic as the lanes' own interfaces
+template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM>
+__device__ __forceinline__ void edge_flux(const typename E::Par &par, int lim, int slim, const Cell<R, E> *c, R dt_dx0,
+	bool fluxOn, R aov, R *Fa, R *dW)
+{
+	constexpr int Q = E::Q;
+	R F[Q];
+	if constexpr (PLM) {
+		Cell<R, E> L1, R1, L2, R2;
+		plm_cons<R, E, SLIM>(slim, c[0], c[1], c[2], L1, R1);
+		plm_cons<R, E, SLIM>(slim, c[1], c[2], c[3], L2, R2);
+		Iface<R, E> I;
+		iface_prep<R, E, 0>(par, R1, L2, I);
+		iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, F);
+#pragma unroll
+		for (int j = 0; j < E::NW; ++j) dW[j] = I.dW[j];
+	} else if constexpr (LIM) {
+		Iface<R, E> Im, I, Ip;
+		iface_prep<R, E, 0>(par, c[0], c[1], Im);
+		iface_prep<R, E, 0>(par, c[1], c[2], I);
+		iface_prep<R, E, 0>(par, c[2], c[3], Ip);
+		iface_flux<R, E, 0, true, FLIM>(par, I, Im.dW, Ip.dW, dt_dx0, lim, F);
+#pragma unroll
+		for (int j = 0; j < E::NW; ++j) dW[j] = I.dW[j];
+	} else {
+		Iface<R, E> I;
+		iface_prep<R, E, 0>(par, c[1], c[2], I);
+		iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, F);
+#pragma unroll
+		for (int j = 0; j < E::NW; ++j) dW[j] = I.dW[j];
+	}
+#pragma unroll
+	for (int j = 0; j < Q; ++j) Fa[j] = fluxOn ? F[j] * aov : R(0);
+}
+
+// LDS stores of a wave, visible to its own lanes' later loads
+__device__ __forceinline__ void wave_lds_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Edge pass of a strip: lane j < NR computes the left edge interface of the
+// wave's row j, lane NR + j the right one (lanes from 2 NR on repeat them and
+// store nothing); cb holds the lane's cells xb-2 .. xb+1 around its
+// interface xb - 1/2.
+template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM, int NR>
+__device__ __forceinline__ void edge_pass(const typename E::Par &par, int lim, int slim, const Cell<R, E> *cb, R dt_dx0,
+	bool fluxOn, R aov, int lane, XEdge<R, E, PLM, LIM, NR> &edge)
+{
+	constexpr int Q = E::Q;
+	R Fa[Q], dW[E::NW];
+	edge_flux<R, E, PLM, LIM, FLIM, SLIM>(par, lim, slim, cb, dt_dx0, fluxOn, aov, Fa, dW);
+	const bool right = (lane & NR) != 0;
+	wave_lds_sync();   // the previous item's reads (persistent waves) are done
+	if (lane < 2 * NR) {
+#pragma unroll
+		for (int j = 0; j < Q; ++j) edge.fa[j][lane] = Fa[j];
+		if constexpr (XEdge<R, E, PLM, LIM, NR>::U) {
+#pragma unroll
+			for (int f = 0; f < Cell<R, E>::NV; ++f) edge.u[f][lane] = right ? cb[2].v[f] : cb[1].v[f];
+		}
+		if constexpr (LIM) {
+#pragma unroll
+			for (int j = 0; j < E::NW; ++j) edge.dw[j][lane] = dW[j];
+		}
+	}
+	wave_lds_sync();
+}
+
+// x part of a row: the x flux divergence (area-weighted, -(F(x+1/2) - F(x-1/2)))
+// of row U, lanes exchanging neighbours by DPP and, with EDGES, the strip
+// edges (lanes 0 and 63) taken from the row's edge slot j (without, lanes 0-1
+// and 62-63 are a halo whose results are not used). Um / Up return the x
+// neighbours.
+template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM, int NR, bool EDGES = true>
+__device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int slim, const Cell<R, E> &U, R dt_dx0,
+	bool fluxOn, R aov, int lane, int j, const XEdge<R, E, PLM, LIM, NR> &edge, Cell<R, E> &Um, Cell<R, E> &Up,
+	R *divx)
+{
+	using C = Cell<R, E>;
+	constexpr int Q = E::Q;
+	const int slot = (lane == WAVE - 1 ? NR : 0) + j;
+	shift_left<R, E>(U, Um);
+	shift_right<R, E>(U, Up);
+	if constexpr (EDGES && XEdge<R, E, PLM, LIM, NR>::U) {
+#pragma unroll
+		for (int f = 0; f < C::NV; ++f) {
+			R e = edge.u[f][slot];
+			Um.v[f] = lane == 0 ? e : Um.v[f];
+			Up.v[f] = lane == WAVE - 1 ? e : Up.v[f];
+		}
+	} else if constexpr (EDGES) {
+		// lane 0's own interface is replaced by the edge's; a valid state
+		// (its own) in place of the zero fill keeps it out of the vacuum paths
+#pragma unroll
+		for (int f = 0; f < C::NV; ++f) Um.v[f] = lane == 0 ? U.v[f] : Um.v[f];
+	}
+	R Fx[Q];
+	if constexpr (PLM) {
+		C Lx, Rx, RxL;
+		plm_cons<R, E, SLIM>(slim, Um, U, Up, Lx, Rx);
+		shift_left<R, E>(Rx, RxL);
+		if constexpr (EDGES) {
+			// as above: lane 0's interface is replaced, its left state only
+			// needs to be valid
+#pragma unroll
+			for (int f = 0; f < C::NV; ++f) RxL.v[f] = lane == 0 ? Rx.v[f] : RxL.v[f];
+		}
+		Iface<R, E> I;
+		iface_prep<R, E, 0>(par, RxL, Lx, I);
+		iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+	} else if constexpr (LIM) {
+		Iface<R, E> I;
+		iface_prep<R, E, 0>(par, Um, U, I);
+		R dWL[E::NW], dWR[E::NW];
+#pragma unroll
+		for (int k = 0; k < E::NW; ++k) {
+			if constexpr (EDGES) {
+				// lane 0's own interface is the left edge one: its dW (from
+				// the cell outside the strip) is lane 1's left neighbour jump
+				R e = edge.dw[k][slot];
+				dWL[k] = from_left(lane == 0 ? e : I.dW[k]);
+				dWR[k] = from_right(I.dW[k]);
+				dWR[k] = lane == WAVE - 1 ? e : dWR[k];
+			} else {
+				dWL[k] = from_left(I.dW[k]);
+				dWR[k] = from_right(I.dW[k]);
+			}
+		}
+		iface_flux<R, E, 0, true, FLIM>(par, I, dWL, dWR, dt_dx0, lim, Fx);
+	} else {
+		Iface<R, E> I;
+		iface_prep<R, E, 0>(par, Um, U, I);
+		iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+	}
+#pragma unroll
+	for (int k = 0; k < Q; ++k) {
+		R Fa = fluxOn ? Fx[k] * aov : R(0);
+		R FaR;
+		if constexpr (EDGES) {
+			R ef = edge.fa[k][slot];
+			Fa = lane == 0 ? ef : Fa;            // the left edge interface
+			FaR = from_right(Fa);
+			FaR = lane == WAVE - 1 ? ef : FaR;   // the right edge interface
+		} else {
+			FaR = from_right(Fa);
+		}
+		divx[k] = R(0) - (FaR - Fa);
+	}
+}
+
 // ------------------------------------------------------- one wave strip
-// Advances the strip (sx, sy) of grid g by one stage: 60 output columns
-// (lanes 2..61) by rowsPerWave rows. Returns the strip's dt minimum of the new
+// Advances the strip (sx, sy) of grid g by one stage: 64 output columns (one
+// per lane; 1D: 60) by rowsPerWave (<= 32) rows. Returns the strip's dt minimum of the new
 // state when the stage reduces dt.
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
 __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridDesc<R> &g, int sx, int sy, double dt,
-	int lane)
+	int lane, XEdge<R, EQ<R>, PLM, LIM, EDGE_ROWS> &edge)
 {
 	using E = EQ<R>;
 	using C = Cell<R, E>;
@@ -186,8 +358,9 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	const R *src = g.U[a.src];
 	R *out = a.out >= 0 ? g.U[a.out] : g.ping;
 	R *storeD = a.storeD >= 0 ? g.D[a.storeD] : nullptr;
-	const int x = sx * STRIP_OUT - NG + lane;
-	const bool outLane = lane >= NG && lane < WAVE - NG && x < nx;
+	constexpr bool EDGES = strip_edges(DIM, E::SOURCE);   // edge pass, else a halo
+	const int x = EDGES ? sx * STRIP_OUT + lane : sx * strip_out(false) - NG + lane;
+	const bool outLane = EDGES ? x < nx : lane >= NG && lane < WAVE - NG && x < nx;
 	const Mapped mx = map_coord(x, nx, g.bc[0], g.bc[1], 0);
 	Mapped myIn;
 	myIn.idx = 0; myIn.mirror = 0; myIn.fixedFace = -1;
@@ -196,6 +369,22 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	const R dt_dx1 = DIM >= 2 ? rdt / g.dx[1] : R(0);
 	const int y0 = sy * g.rowsPerWave;
 	const int y1 = min(y0 + g.rowsPerWave, ny);
+
+	// ---- the strip's edge interfaces for rows y0 .. y1-1: lane j (< 32) the
+	// left edge of row y0 + j, lane 32 + j the right edge
+	if constexpr (EDGES) {
+		const int bj = lane & (EDGE_ROWS - 1);
+		const bool right = lane >= EDGE_ROWS;
+		const int yb = y0 + (bj < y1 - y0 ? bj : 0);
+		const int xb = sx * STRIP_OUT + (right ? STRIP_OUT : 0);   // interface xb - 1/2
+		const Mapped myb = DIM >= 2 ? map_coord(yb, ny, g.bc[2], g.bc[3], 2) : myIn;
+		C cb[4];
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			load_cell<R, E>(g, src, map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0), myb, cb[k]);
+		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS>(par, lim, slim, cb, dt_dx0, g.fluxOn[0], g.areaOverVol[0],
+			lane, edge);
+	}
 
 	// y pipeline state
 	C Wm, W0, W1, W2;       // rows r-1 .. r+2
@@ -280,46 +469,20 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			C U;
 			if constexpr (DIM >= 2) U = W0;
 			else loadRow(0, U);
-			R Fx[Q];
-			if constexpr (PLM) {
-				C Um, Up, Lx, Rx, RxL;
-				shift_left<R, E>(U, Um);
-				shift_right<R, E>(U, Up);
-				plm_cons<R, E, SLIM>(slim, Um, U, Up, Lx, Rx);
-				shift_left<R, E>(Rx, RxL);
-				Iface<R, E> I;
-				iface_prep<R, E, 0>(par, RxL, Lx, I);
-				iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
-			} else if constexpr (LIM) {
-				C Um;
-				shift_left<R, E>(U, Um);
-				Iface<R, E> I;
-				iface_prep<R, E, 0>(par, Um, U, I);
-				R dWL[E::NW], dWR[E::NW];
-#pragma unroll
-				for (int j = 0; j < E::NW; ++j) { dWL[j] = from_left(I.dW[j]); dWR[j] = from_right(I.dW[j]); }
-				iface_flux<R, E, 0, true, FLIM>(par, I, dWL, dWR, dt_dx0, lim, Fx);
-			} else {
-				C Um;
-				shift_left<R, E>(U, Um);
-				Iface<R, E> I;
-				iface_prep<R, E, 0>(par, Um, U, I);
-				iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
-			}
+			C Um, Up;
 			R deriv[Q];
+			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS, EDGES>(par, lim, slim, U, dt_dx0, g.fluxOn[0],
+				g.areaOverVol[0], lane, r - y0, edge, Um, Up, deriv);
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
-				R Fa = g.fluxOn[0] ? Fx[j] * g.areaOverVol[0] : R(0);
-				R FaR = from_right(Fa);
-				deriv[j] = R(0) - (FaR - Fa);
 				if constexpr (DIM >= 2) deriv[j] = deriv[j] - dyterm[j];
 				if (!g.derivOn) deriv[j] = R(0);
 			}
 			if constexpr (E::SOURCE) {
 				// eqn addSource after calcDerivFromFlux (solverbase.lua:3208-3214)
 				R em[3], ep[3];
-				em[0] = from_left(U.v[10]);
-				ep[0] = from_right(U.v[10]);
+				em[0] = Um.v[10];
+				ep[0] = Up.v[10];
 				if constexpr (DIM >= 2) { em[1] = Wm.v[10]; ep[1] = W1.v[10]; }
 				E::template source<DIM>(par, g.invdx, U.v, em, ep, deriv);
 			}
@@ -425,15 +588,20 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
 	const int waveInBlock = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const long item = (long)blk * (blockDim.x >> 6) + waveInBlock;
+	__shared__ XEdge<R, EQ<R>, PLM, LIM, EDGE_ROWS> edges[strip_edges(DIM, EQ<R>::SOURCE) ? 4 : 1];
 	const GridDesc<R> &g = a.g;
 	double dt = stage_dt(a, g);
 	if (item >= g.nItems) return;     // whole waves leave; no barriers follow
 	const int sx = (int)(item % g.stripsX);
 	const int sy = (int)(item / g.stripsX);
 	if (item == 0) stage_first_strip(a, g, dt, lane);
-	double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM>(a, g, sx, sy, dt, lane);
+	double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM>(a, g, sx, sy, dt, lane, edges[waveInBlock]);
 	stage_reduce(a, g, dtLocal, item, lane);
 }
+
+// Dynamic LDS of the batched kernel: the strip-edge slices of its waves
+// (waves per block x sizeof(XEdge), set at launch; none for halo strips).
+extern __shared__ __align__(16) unsigned char batch_lds[];
 
 // Batched form: a persistent launch over every strip of every active grid of
 // the batch; wave k takes items k, k + W, k + 2W, ... (W = waves launched).
@@ -447,6 +615,7 @@ __global__ void __launch_bounds__(256) stage_kernel_batch(const StageArgs<R> a)
 	const int waveInBlock = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const long wave = (long)blk * (blockDim.x >> 6) + waveInBlock;
 	const long nWaves = (long)gridDim.x * (blockDim.x >> 6);
+	auto *edges = reinterpret_cast<XEdge<R, EQ<R>, PLM, LIM, EDGE_ROWS> *>(batch_lds);
 	int gi = 0;
 	for (long item = wave; item < a.totalItems; item += nWaves) {
 		// grids are ordered by itemBase; items only increase, so search upward
@@ -464,7 +633,7 @@ __global__ void __launch_bounds__(256) stage_kernel_batch(const StageArgs<R> a)
 		const int sx = (int)(local % g.stripsX);
 		const int sy = (int)(local / g.stripsX);
 		if (local == 0) stage_first_strip(a, g, dt, lane);
-		double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM>(a, g, sx, sy, dt, lane);
+		double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM>(a, g, sx, sy, dt, lane, edges[waveInBlock]);
 		stage_reduce(a, g, dtLocal, local, lane);
 	}
 }

@@ -4,9 +4,11 @@
 // multAdd... -> constrainU, and the new state's dt on the last stage;
 // fvsolver.lua:225-302, rk.lua:152-183, calcDT.cl:38-73), with a third axis.
 //
-// Work decomposition: a block of W waves owns a 60-column strip (x) of W
-// consecutive rows (y) and marches a chunk of planes in z. In every wave:
-//   x neighbours: DPP lane shifts (as in 2D);
+// Work decomposition: a block of W waves owns a 64-column strip (x) of W
+// consecutive rows (y) and marches a chunk of at most 16 planes in z. In
+// every wave:
+//   x neighbours: DPP lane shifts, and the strip's edge interfaces of all the
+//     chunk's planes computed in one pass before the planes (as in 2D);
 //   z neighbours: a sliding register window of planes (as y in 2D);
 //   y neighbours: the block's rows of the current plane, exchanged through
 //     LDS. Each interface in y is computed once per block: wave w takes the
@@ -76,7 +78,7 @@ __device__ __forceinline__ void lds_get(const R (&s)[NV][WAVE], R *v, int lane) 
 
 template <typename R, template <typename> class EQ, bool PLM, bool LIM, int FLIM, int SLIM, int W>
 __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const GridDesc<R> &g, int sx, int by, int bz,
-	int w, double dt, int lane, Lds3<R, EQ<R>, W> &sh)
+	int w, double dt, int lane, Lds3<R, EQ<R>, W> &sh, XEdge<R, EQ<R>, PLM, LIM, EDGE_PLANES> &edge)
 {
 	using E = EQ<R>;
 	using C = Cell<R, E>;
@@ -94,10 +96,10 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	const R *src = g.U[a.src];
 	R *out = a.out >= 0 ? g.U[a.out] : g.ping;
 	R *storeD = a.storeD >= 0 ? g.D[a.storeD] : nullptr;
-	const int x = sx * STRIP_OUT - NG + lane;
+	const int x = sx * STRIP_OUT + lane;
 	const int y0 = by * W;
 	const int y = y0 + w;
-	const bool outLane = lane >= NG && lane < WAVE - NG && x < nx && y < ny;
+	const bool outLane = x < nx && y < ny;
 	const Mapped mx = map_coord(x, nx, g.bc[0], g.bc[1], 0);
 	const Mapped myOwn = map_coord(y, ny, g.bc[2], g.bc[3], 2);
 	const R rdt = R(dt);
@@ -116,6 +118,22 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 		Mapped mz = map_coord(zz, nz, g.bc[4], g.bc[5], 4);
 		load_cell3<R, E>(g, src, mx, myOwn, mz, c);
 	};
+
+	// ---- the strip's edge interfaces of row y for planes z0 .. z1-1: lane j
+	// (< 16) the left edge of plane z0 + j, lane 16 + j the right edge
+	{
+		const int bj = lane & (EDGE_PLANES - 1);
+		const bool right = (lane & EDGE_PLANES) != 0;
+		const int zb = z0 + (bj < z1 - z0 ? bj : 0);
+		const int xb = sx * STRIP_OUT + (right ? STRIP_OUT : 0);   // interface xb - 1/2
+		const Mapped mz = map_coord(zb, nz, g.bc[4], g.bc[5], 4);
+		C cb[4];
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			load_cell3<R, E>(g, src, map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0), myOwn, mz, cb[k]);
+		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_PLANES>(par, lim, slim, cb, dt_dx0, g.fluxOn[0], g.areaOverVol[0],
+			lane, edge);
+	}
 
 	// z pipeline state (the 2D kernel's y pipeline)
 	C W0, W1, W2;
@@ -309,38 +327,12 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			for (int j = 0; j < Q; ++j) dyterm[j] = sh.f[w + 1][j][lane] - FaY[j];
 
 			// ---- x direction (lanes)
-			R Fx[Q];
-			if constexpr (PLM) {
-				C Um, Up, Lx, Rx, RxL;
-				shift_left<R, E>(U, Um);
-				shift_right<R, E>(U, Up);
-				plm_cons<R, E, SLIM>(slim, Um, U, Up, Lx, Rx);
-				shift_left<R, E>(Rx, RxL);
-				Iface<R, E> I;
-				iface_prep<R, E, 0>(par, RxL, Lx, I);
-				iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
-			} else if constexpr (LIM) {
-				C Um;
-				shift_left<R, E>(U, Um);
-				Iface<R, E> I;
-				iface_prep<R, E, 0>(par, Um, U, I);
-				R dWL[E::NW], dWR[E::NW];
-#pragma unroll
-				for (int j = 0; j < E::NW; ++j) { dWL[j] = from_left(I.dW[j]); dWR[j] = from_right(I.dW[j]); }
-				iface_flux<R, E, 0, true, FLIM>(par, I, dWL, dWR, dt_dx0, lim, Fx);
-			} else {
-				C Um;
-				shift_left<R, E>(U, Um);
-				Iface<R, E> I;
-				iface_prep<R, E, 0>(par, Um, U, I);
-				iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
-			}
+			C Um, Up;
 			R deriv[Q];
+			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_PLANES>(par, lim, slim, U, dt_dx0, g.fluxOn[0],
+				g.areaOverVol[0], lane, r - z0, edge, Um, Up, deriv);
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
-				R Fa = g.fluxOn[0] ? Fx[j] * g.areaOverVol[0] : R(0);
-				R FaR = from_right(Fa);
-				deriv[j] = R(0) - (FaR - Fa);
 				deriv[j] = deriv[j] - dyterm[j];
 				deriv[j] = deriv[j] - dzterm[j];
 				if (!g.derivOn) deriv[j] = R(0);
@@ -414,6 +406,7 @@ __global__ void __launch_bounds__(WAVE * Block3<R>::W) stage_kernel3d(const Stag
 {
 	constexpr int W = Block3<R>::W;
 	__shared__ Lds3<R, EQ<R>, W> sh;
+	__shared__ XEdge<R, EQ<R>, PLM, LIM, EDGE_PLANES> edges[W];
 	const int lane = threadIdx.x & 63;
 	const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
@@ -424,7 +417,7 @@ __global__ void __launch_bounds__(WAVE * Block3<R>::W) stage_kernel3d(const Stag
 	const int by = rest % g.stripsY;
 	const int bz = rest / g.stripsY;
 	if (blk == 0 && w == 0) stage_first_strip(a, g, dt, lane);
-	double dtLocal = stage_block3d<R, EQ, PLM, LIM, FLIM, SLIM, W>(a, g, sx, by, bz, w, dt, lane, sh);
+	double dtLocal = stage_block3d<R, EQ, PLM, LIM, FLIM, SLIM, W>(a, g, sx, by, bz, w, dt, lane, sh, edges[w]);
 	stage_reduce(a, g, dtLocal, (long)blk * W + w, lane);
 }
 

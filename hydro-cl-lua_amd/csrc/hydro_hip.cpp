@@ -108,7 +108,9 @@ extern "C" size_t hydro_device_bytes(const hydro_solver *h) { return h ? h->devB
 
 extern "C" int hydro_set_launch(hydro_solver *h, int rows_per_wave, int xcd_remap) {
 	if (!h) return fail("NULL handle");
-	if (rows_per_wave < 0 || rows_per_wave > 4096) return fail("rows_per_wave %d out of range", rows_per_wave);
+	const int maxRows = h->g.dim == 3 ? EDGE_PLANES : EDGE_ROWS;
+	if (rows_per_wave < 0 || rows_per_wave > maxRows)
+		return fail("rows_per_wave %d out of range 0..%d", rows_per_wave, maxRows);
 	if (rows_per_wave > 0 && h->g.dim >= 2) h->g.rowsPerWave = rows_per_wave;
 	if (xcd_remap > 0) h->xcdRemap = xcd_remap == 1;
 	return 0;
@@ -341,7 +343,8 @@ static long batch_items(const hydro_batch *b) {
 	long n = 0;
 	for (auto &g : b->grids) {
 		int rows = g.dim >= 2 ? b->rowsPerWave : 1;
-		long sx = (g.n[0] + STRIP_OUT - 1) / STRIP_OUT;
+		const int so = strip_out(strip_edges(g.dim, g.cfg.eqn == HYDRO_EQN_GLM_MAXWELL));
+		long sx = (g.n[0] + so - 1) / so;
 		long sy = g.dim >= 2 ? (g.n[1] + rows - 1) / rows : 1;
 		n += sx * sy;
 	}
@@ -411,7 +414,7 @@ extern "C" int hydro_batch_create(const hydro_cfg *cfgs, int n, hydro_batch **ou
 		grid_init(b->grids[i], cfgs[i]);
 		cells += b->grids[i].ncell;
 	}
-	b->rowsPerWave = cfgs[0].dim >= 2 ? (int)std::max(4L, std::min(32L, cells / (60L * 4096) + 1)) : 1;
+	b->rowsPerWave = cfgs[0].dim >= 2 ? (int)std::max(4L, std::min(32L, cells / (64L * 4096) + 1)) : 1;
 	b->active.assign(n, 1);
 	auto cleanup = [&](int rc) { hydro_batch_destroy(b); return rc; };
 	if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -495,7 +498,8 @@ extern "C" int hydro_batch_set_active(hydro_batch *b, int i, int active) {
 
 extern "C" int hydro_batch_set_launch(hydro_batch *b, int rows_per_wave, int waves_per_block) {
 	if (!b) return fail("NULL handle");
-	if (rows_per_wave < 0 || rows_per_wave > 4096) return fail("rows_per_wave %d out of range", rows_per_wave);
+	if (rows_per_wave < 0 || rows_per_wave > EDGE_ROWS)
+		return fail("rows_per_wave %d out of range 0..%d", rows_per_wave, EDGE_ROWS);
 	if (waves_per_block != 0 && waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
 		return fail("waves_per_block must be 1, 2 or 4");
 	if (rows_per_wave > 0) b->rowsPerWave = rows_per_wave;

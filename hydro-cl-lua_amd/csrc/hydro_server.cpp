@@ -373,6 +373,8 @@ extern "C" int hydro_server_info_get(hydro_server *s, hydro_server_info *info) {
 
 extern "C" int hydro_server_set_launch(hydro_server *s, int rows_per_wave, int waves_per_block) {
 	if (!s) return fail("NULL handle");
+	if (rows_per_wave < 0 || rows_per_wave > EDGE_ROWS)
+		return fail("rows_per_wave %d out of range 0..%d", rows_per_wave, EDGE_ROWS);
 	if (waves_per_block != 0 && waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
 		return fail("waves_per_block must be 1, 2 or 4");
 	if (rows_per_wave > 0) s->rowsPerWave = rows_per_wave;

@@ -65,12 +65,15 @@ class Server:
         _abi.check(_abi.lib().hydro_server_set_launch(self.h, int(rows), int(wavesPerBlock)))
 
 
-def serve(requests, budget_bytes, steps_per_tick=20, keep_results=False, closed_loop=False):
+def serve(requests, budget_bytes, steps_per_tick=20, keep_results=False, closed_loop=False, launch=None):
     """Runs the arrival stream against a Server in wall-clock time. Each
     request is submitted when its arrival time comes (all at once when
     closed_loop); returns per-request latency (arrival -> result on the host),
-    the makespan, and optionally the results by request index."""
+    the makespan, and optionally the results by request index. launch =
+    (rows per wave, waves per block) sets the pools' launch shape."""
     srv = Server(budget_bytes, steps_per_tick)
+    if launch:
+        srv.setLaunch(*launch)
     states = [r.initial_state() for r in requests]
     order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
     lat = [None] * len(requests)

@@ -207,9 +207,9 @@ int hydro_time_updates(hydro_solver *h, int nsteps, int per_kernel_events, hydro
 
 /* Launch shape of the fused stage kernel (no counterpart in the reference,
  * whose work sizes are fixed at gridsolver.lua:141-243): rows each 64-lane
- * wave marches in y, and whether blocks are remapped so that each XCD works on
- * a contiguous band of strips. 0 keeps the current value. The autotuner sets
- * these; results do not depend on them. */
+ * wave marches, 1..32 (3D: planes per block, 1..16), and whether blocks are remapped so that
+ * each XCD works on a contiguous band of strips. 0 keeps the current value.
+ * Results do not depend on them. */
 int hydro_set_launch(hydro_solver *h, int rows_per_wave, int xcd_remap);
 int hydro_get_launch(const hydro_solver *h, int *rows_per_wave, int *xcd_remap);
 
@@ -221,7 +221,7 @@ size_t hydro_device_bytes(const hydro_solver *h);
 /* Many independent grids advanced together: the reference runs several solvers
  * from one app loop, each with its own update (hydro/app.lua:1573-1737); here
  * every stage of every grid of the batch is one persistent launch (one wave per
- * 60-column strip, strips of all grids dealt to the waves in turn), each grid
+ * 64-column strip, strips of all grids dealt to the waves in turn), each grid
  * keeps its own dt (cfl * its own min, solverbase.lua:3004-3023) and time, and
  * three steps (one turn of the dt-slot rotation) can be replayed from a
  * captured hipGraph. All grids share eqn, dim, flux, limiters, plm, integrator,
@@ -246,7 +246,7 @@ int hydro_batch_sync(hydro_batch *b);
 /* device time of nsteps updates (HIP events on the batch stream) */
 int hydro_batch_time_updates(hydro_batch *b, int nsteps, int use_graph, double *ms);
 /* Launch shape (no counterpart in the reference): rows each wave marches in y
- * and waves per block (1, 2, 4); 0 keeps the current value. */
+ * (1..32) and waves per block (1, 2, 4); 0 keeps the current value. */
 int hydro_batch_set_launch(hydro_batch *b, int rows_per_wave, int waves_per_block);
 int hydro_batch_get_launch(const hydro_batch *b, int *rows_per_wave, int *waves_per_block);
 /* Autotuner (no counterpart in the reference): times trial_steps updates for
@@ -292,8 +292,8 @@ int hydro_server_poll(hydro_server *s, long long *ids, int max, int *n);
 int hydro_server_result(hydro_server *s, long long id, void *U, size_t bytes, double *t, int *status);
 int hydro_server_info_get(hydro_server *s, hydro_server_info *info);
 /* launch shape of the pools, present and future (default: 12 rows per wave
- * for 2D pools, one wave per block); 0 keeps a value. No counterpart in the
- * reference. */
+ * for 2D pools, one wave per block; rows 1..32); 0 keeps a value. No
+ * counterpart in the reference. */
 int hydro_server_set_launch(hydro_server *s, int rows_per_wave, int waves_per_block);
 
 #ifdef __cplusplus

@@ -104,12 +104,21 @@ def test_3d_fixed_faces_and_forward_euler():
     assert np.all(err <= TOL64), err
 
 
-@pytest.mark.parametrize('rows', [4, 7, 64])
+@pytest.mark.parametrize("rows", [4, 7, 16])
 def test_3d_plane_chunks_do_not_change_bits(rows):
     """the plane chunk per block changes the decomposition, not the numbers"""
     err, s, o = _run(_octants(4), 4, size=(21, 11, 30), boundary='periodic', flux_limiter='van Leer',
                      integrator='Runge-Kutta 2, TVD', cfl=.3, rows=rows)
     assert np.all(err <= TOL64), err
+
+
+def test_3d_plane_chunk_limit():
+    from hydro._abi import HydroError
+    from hydro.solver.hipfvsolver import HipFVSolver
+    s = HipFVSolver(dim=3, gridSize=(8, 8, 8), integrator='forward Euler', cfl=.3, boundary='periodic')
+    s.setLaunch(16, 0)
+    with pytest.raises(HydroError):
+        s.setLaunch(17, 0)
 
 
 def test_3d_fast_math():

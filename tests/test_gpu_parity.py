@@ -109,6 +109,23 @@ def test_quadrant_2d_flux_limiter(lim):
     _check(s, o, 2, TOL64, 10, f'fluxlim-{lim}')
 
 
+@pytest.mark.parametrize('rows', [1, 3, 32])
+@pytest.mark.parametrize('mode', ['plm', 'flux limiter', 'donor cell'])
+def test_strip_edges_2d(rows, mode):
+    """64-column strips take their two edge interfaces from a per-row-block pass:
+    three strips (the last one partial), each reconstruction, rows per wave
+    1..32, mirror faces in x"""
+    from hydro._abi import HydroError
+    rec = {'plm': dict(usePLM='plm cons', slopeLimiter='van Leer'), 'flux limiter': dict(fluxLimiter='superbee'),
+           'donor cell': {}}[mode]
+    kw = dict(dim=2, gridSize=(150, 45), integrator='Runge-Kutta 2, TVD', cfl=.3, boundary='mirror', **rec)
+    s, o = _pair(_quad(5), 8, **kw)
+    with pytest.raises(HydroError):
+        s.setLaunch(33, 0)
+    s.setLaunch(rows, 0)
+    _check(s, o, 2, TOL64, 8, f'edges-{mode}-{rows}')
+
+
 @pytest.mark.parametrize('integ', ['forward Euler', 'Runge-Kutta 2', 'Runge-Kutta 2 Heun', 'Runge-Kutta 2 Ralston',
                                    'Runge-Kutta 3', 'Runge-Kutta 4', 'Runge-Kutta 4, 3/8ths rule',
                                    'Runge-Kutta 2, non-TVD', 'Runge-Kutta 4, TVD', 'Runge-Kutta 4, non-TVD'])

@@ -1,6 +1,9 @@
-// FP64 kernels. Built with -ffp-contract=off so the arithmetic is the
-// reference's expression order, operation for operation.
+// FP64 kernels of the EXACT policy, the dispatch and the auxiliary kernels.
+// Built with -ffp-contract=off so the arithmetic is the reference's expression
+// order, operation for operation.
 #define REAL double
+#define STAGE_EXACT
+#define AUX_KERNELS
 #include "fv_kernels.inc"
 
 namespace hydro {

@@ -142,6 +142,29 @@ __device__ __forceinline__ float from_right(float v) {
 	return __builtin_bit_cast(float, dpp_from_right_i(__builtin_bit_cast(int, v)));
 }
 
+// The same shifts with the empty lane (0 / 63) taking `edge` instead of 0:
+// the DPP move's old operand, so no select is needed.
+__device__ __forceinline__ double from_left_or(double v, double edge) {
+	int2 p = __builtin_bit_cast(int2, v), e = __builtin_bit_cast(int2, edge);
+	p.x = __builtin_amdgcn_update_dpp(e.x, p.x, 0x138, 0xF, 0xF, false);
+	p.y = __builtin_amdgcn_update_dpp(e.y, p.y, 0x138, 0xF, 0xF, false);
+	return __builtin_bit_cast(double, p);
+}
+__device__ __forceinline__ double from_right_or(double v, double edge) {
+	int2 p = __builtin_bit_cast(int2, v), e = __builtin_bit_cast(int2, edge);
+	p.x = __builtin_amdgcn_update_dpp(e.x, p.x, 0x130, 0xF, 0xF, false);
+	p.y = __builtin_amdgcn_update_dpp(e.y, p.y, 0x130, 0xF, 0xF, false);
+	return __builtin_bit_cast(double, p);
+}
+__device__ __forceinline__ float from_left_or(float v, float edge) {
+	return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+		__builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right_or(float v, float edge) {
+	return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+		__builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+
 // ------------------------------------------------------- boundary mapping
 // A ghost coordinate maps to an interior source cell, a sign for mirror
 // reflection, or a fixed value (gridsolver.lua:630-780). The reference fills

@@ -268,16 +268,20 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 	using C = Cell<R, E>;
 	constexpr int Q = E::Q;
 	const int slot = (lane == WAVE - 1 ? NR : 0) + j;
-	shift_left<R, E>(U, Um);
-	shift_right<R, E>(U, Up);
 	if constexpr (EDGES && XEdge<R, E, PLM, LIM, NR>::U) {
+		// lane 0 reads the left edge slot, lane 63 the right one: the shifts'
+		// empty lanes take them
 #pragma unroll
 		for (int f = 0; f < C::NV; ++f) {
 			R e = edge.u[f][slot];
-			Um.v[f] = lane == 0 ? e : Um.v[f];
-			Up.v[f] = lane == WAVE - 1 ? e : Up.v[f];
+			Um.v[f] = from_left_or(U.v[f], e);
+			Up.v[f] = from_right_or(U.v[f], e);
 		}
-	} else if constexpr (EDGES) {
+	} else {
+		shift_left<R, E>(U, Um);
+		shift_right<R, E>(U, Up);
+	}
+	if constexpr (EDGES && !XEdge<R, E, PLM, LIM, NR>::U) {
 		// lane 0's own interface is replaced by the edge's; a valid state
 		// (its own) in place of the zero fill keeps it out of the vacuum paths
 #pragma unroll
@@ -308,8 +312,7 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 				// the cell outside the strip) is lane 1's left neighbour jump
 				R e = edge.dw[k][slot];
 				dWL[k] = from_left(lane == 0 ? e : I.dW[k]);
-				dWR[k] = from_right(I.dW[k]);
-				dWR[k] = lane == WAVE - 1 ? e : dWR[k];
+				dWR[k] = from_right_or(I.dW[k], e);
 			} else {
 				dWL[k] = from_left(I.dW[k]);
 				dWR[k] = from_right(I.dW[k]);
@@ -326,10 +329,9 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 		R Fa = fluxOn ? Fx[k] * aov : R(0);
 		R FaR;
 		if constexpr (EDGES) {
-			R ef = edge.fa[k][slot];
-			Fa = lane == 0 ? ef : Fa;            // the left edge interface
-			FaR = from_right(Fa);
-			FaR = lane == WAVE - 1 ? ef : FaR;   // the right edge interface
+			R ef = edge.fa[k][slot];             // lane 0: left edge, lane 63: right edge
+			Fa = lane == 0 ? ef : Fa;
+			FaR = from_right_or(Fa, ef);
 		} else {
 			FaR = from_right(Fa);
 		}

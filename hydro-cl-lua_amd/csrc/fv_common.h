@@ -65,6 +65,7 @@ struct GridDesc {
 	R areaOverVol[3];         // areaL * (1 / volume) per side (fvsolver.cl:64-102)
 	int derivOn;              // volume > 1e-7 (fvsolver.cl:97)
 	int fluxOn[3];            // cell_area > 1e-7 (fvsolver.lua:105)
+	R aovFast[3];             // FAST kernels: areaOverVol where flux and derivative are on, else 0
 	R dx[3];
 	R invdx[3];               // 1 / dx in the stored precision (glm-maxwell.cl:351)
 	double dxD[3];

@@ -339,6 +339,9 @@ static void grid_desc(const Grid &g, GridDesc<R> &d) {
 		if (aL <= R(1e-7)) aL = R(0);
 		d.areaOverVol[side] = aL * invVolume;
 		d.fluxOn[side] = area > R(1e-7);
+		// both guards as one factor: the flux term is then exactly 0 where a
+		// guard is off (for finite fluxes; the sign of a zero may differ)
+		d.aovFast[side] = d.fluxOn[side] && d.derivOn ? d.areaOverVol[side] : R(0);
 	}
 	d.cfl = c.cfl;
 	d.rowsPerWave = g.rowsPerWave;

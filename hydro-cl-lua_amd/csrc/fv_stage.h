@@ -62,6 +62,19 @@ __device__ __forceinline__ void load_cell(const GridDesc<R> &g, const R *base, c
 	}
 }
 
+// A cell whose coordinates map to the interior or through periodic faces
+// only: no fixed values or mirror flips (the common case, taken by a
+// wave-uniform branch around load_cell).
+template <typename R, class E>
+__device__ __forceinline__ void load_plain(const GridDesc<R> &g, const R *base, long off, Cell<R, E> &c) {
+#pragma unroll
+	for (int f = 0; f < Cell<R, E>::NV; ++f) {
+		if (!E::active(f)) c.v[f] = R(0);
+		else if (f < E::Q) c.v[f] = base[f * g.ps + off];
+		else c.v[f] = g.stat[(f - E::Q) * g.ps + off];
+	}
+}
+
 template <typename R, class E>
 __device__ __forceinline__ void shift_left(const Cell<R, E> &in, Cell<R, E> &out) {
 #pragma unroll
@@ -167,6 +180,20 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 	E::template right<S>(p, I.e, fluxEig, Y);
 #pragma unroll
 	for (int j = 0; j < E::Q; ++j) F[j] = Y[j] + I.Favg[j];
+}
+
+// ------------------------------------------------------------ flux guards
+// The reference zeroes a flux where the cell area is <= 1e-7 and the whole
+// derivative where the volume is (fvsolver.lua:105, fvsolver.cl:97). EXACT
+// kernels apply both as selects; FAST kernels fold them into one factor per
+// side (GridDesc::aovFast), one multiply and no selects.
+template <class E, typename R>
+__device__ __forceinline__ bool flux_on(const GridDesc<R> &g, int side) {
+	return E::FASTM ? true : g.fluxOn[side] != 0;
+}
+template <class E, typename R>
+__device__ __forceinline__ R flux_aov(const GridDesc<R> &g, int side) {
+	return E::FASTM ? g.aovFast[side] : g.areaOverVol[side];
 }
 
 // ------------------------------------------------------------ strip edges
@@ -384,7 +411,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 #pragma unroll
 		for (int k = 0; k < 4; ++k)
 			load_cell<R, E>(g, src, map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0), myb, cb[k]);
-		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS>(par, lim, slim, cb, dt_dx0, g.fluxOn[0], g.areaOverVol[0],
+		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS>(par, lim, slim, cb, dt_dx0, flux_on<E>(g, 0), flux_aov<E>(g, 0),
 			lane, edge);
 	}
 
@@ -402,9 +429,12 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	for (int j = 0; j < E::NW; ++j) dW0[j] = R(0);
 	I1 = Iface<R, E>{};
 
+	// the wave's x coordinates need no fixed values or mirror flips
+	const bool xPlain = __all(mx.fixedFace < 0 && !mx.mirror);
 	auto loadRow = [&](int r, C &c) {
 		Mapped my = DIM >= 2 ? map_coord(r, ny, g.bc[2], g.bc[3], 2) : myIn;
-		load_cell<R, E>(g, src, mx, my, c);
+		if (xPlain && my.fixedFace < 0 && !my.mirror) load_plain<R, E>(g, src, (long)my.idx * g.step1 + mx.idx, c);
+		else load_cell<R, E>(g, src, mx, my, c);
 	};
 
 	// iterations before y0 only fill the pipeline
@@ -460,7 +490,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			}
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
-				R FaY = g.fluxOn[1] ? F[j] * g.areaOverVol[1] : R(0);
+				R FaY = flux_on<E>(g, 1) ? F[j] * flux_aov<E>(g, 1) : R(0);
 				dyterm[j] = FaY - FaYprev[j];
 				FaYprev[j] = FaY;
 			}
@@ -473,12 +503,12 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			else loadRow(0, U);
 			C Um, Up;
 			R deriv[Q];
-			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS, EDGES>(par, lim, slim, U, dt_dx0, g.fluxOn[0],
-				g.areaOverVol[0], lane, r - y0, edge, Um, Up, deriv);
+			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS, EDGES>(par, lim, slim, U, dt_dx0, flux_on<E>(g, 0),
+				flux_aov<E>(g, 0), lane, r - y0, edge, Um, Up, deriv);
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
 				if constexpr (DIM >= 2) deriv[j] = deriv[j] - dyterm[j];
-				if (!g.derivOn) deriv[j] = R(0);
+				if (!E::FASTM && !g.derivOn) deriv[j] = R(0);
 			}
 			if constexpr (E::SOURCE) {
 				// eqn addSource after calcDerivFromFlux (solverbase.lua:3208-3214)

@@ -114,9 +114,13 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 		Mapped mz = map_coord(zz, nz, g.bc[4], g.bc[5], 4);
 		load_cell3<R, E>(g, src, mx, my, mz, c);
 	};
+	// the wave's x coordinates and row need no fixed values or mirror flips
+	const bool xyPlain = __all(mx.fixedFace < 0 && !mx.mirror) && myOwn.fixedFace < 0 && !myOwn.mirror;
 	auto loadPlane = [&](int zz, C &c) {
 		Mapped mz = map_coord(zz, nz, g.bc[4], g.bc[5], 4);
-		load_cell3<R, E>(g, src, mx, myOwn, mz, c);
+		if (xyPlain && mz.fixedFace < 0 && !mz.mirror)
+			load_plain<R, E>(g, src, ((long)mz.idx * g.n[1] + myOwn.idx) * g.step1 + mx.idx, c);
+		else load_cell3<R, E>(g, src, mx, myOwn, mz, c);
 	};
 
 	// ---- the strip's edge interfaces of row y for planes z0 .. z1-1: lane j
@@ -131,8 +135,8 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 #pragma unroll
 		for (int k = 0; k < 4; ++k)
 			load_cell3<R, E>(g, src, map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0), myOwn, mz, cb[k]);
-		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_PLANES>(par, lim, slim, cb, dt_dx0, g.fluxOn[0], g.areaOverVol[0],
-			lane, edge);
+		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_PLANES>(par, lim, slim, cb, dt_dx0, flux_on<E>(g, 0),
+			flux_aov<E>(g, 0), lane, edge);
 	}
 
 	// z pipeline state (the 2D kernel's y pipeline)
@@ -205,7 +209,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			}
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
-				R FaZ = g.fluxOn[2] ? F[j] * g.areaOverVol[2] : R(0);
+				R FaZ = flux_on<E>(g, 2) ? F[j] * flux_aov<E>(g, 2) : R(0);
 				dzterm[j] = FaZ - FaZprev[j];
 				FaZprev[j] = FaZ;
 			}
@@ -317,9 +321,9 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			R FaY[Q];
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
-				FaY[j] = g.fluxOn[1] ? Fy[j] * g.areaOverVol[1] : R(0);
+				FaY[j] = flux_on<E>(g, 1) ? Fy[j] * flux_aov<E>(g, 1) : R(0);
 				sh.f[w][j][lane] = FaY[j];
-				if (w == W - 1) sh.f[W][j][lane] = g.fluxOn[1] ? FyTop[j] * g.areaOverVol[1] : R(0);
+				if (w == W - 1) sh.f[W][j][lane] = flux_on<E>(g, 1) ? FyTop[j] * flux_aov<E>(g, 1) : R(0);
 			}
 			__syncthreads();
 			R dyterm[Q];
@@ -329,13 +333,13 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			// ---- x direction (lanes)
 			C Um, Up;
 			R deriv[Q];
-			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_PLANES>(par, lim, slim, U, dt_dx0, g.fluxOn[0],
-				g.areaOverVol[0], lane, r - z0, edge, Um, Up, deriv);
+			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_PLANES>(par, lim, slim, U, dt_dx0, flux_on<E>(g, 0),
+				flux_aov<E>(g, 0), lane, r - z0, edge, Um, Up, deriv);
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
 				deriv[j] = deriv[j] - dyterm[j];
 				deriv[j] = deriv[j] - dzterm[j];
-				if (!g.derivOn) deriv[j] = R(0);
+				if (!E::FASTM && !g.derivOn) deriv[j] = R(0);
 			}
 
 			// ---- integrator combine, constrainU, new state's dt

@@ -407,10 +407,21 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 		const int yb = y0 + (bj < y1 - y0 ? bj : 0);
 		const int xb = sx * STRIP_OUT + (right ? STRIP_OUT : 0);   // interface xb - 1/2
 		const Mapped myb = DIM >= 2 ? map_coord(yb, ny, g.bc[2], g.bc[3], 2) : myIn;
-		C cb[4];
+		Mapped mxb[4];
+		bool plain = myb.fixedFace < 0 && !myb.mirror;
 #pragma unroll
-		for (int k = 0; k < 4; ++k)
-			load_cell<R, E>(g, src, map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0), myb, cb[k]);
+		for (int k = 0; k < 4; ++k) {
+			mxb[k] = map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0);
+			plain = plain && mxb[k].fixedFace < 0 && !mxb[k].mirror;
+		}
+		C cb[4];
+		if (__all(plain)) {
+#pragma unroll
+			for (int k = 0; k < 4; ++k) load_plain<R, E>(g, src, (long)myb.idx * g.step1 + mxb[k].idx, cb[k]);
+		} else {
+#pragma unroll
+			for (int k = 0; k < 4; ++k) load_cell<R, E>(g, src, mxb[k], myb, cb[k]);
+		}
 		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS>(par, lim, slim, cb, dt_dx0, flux_on<E>(g, 0), flux_aov<E>(g, 0),
 			lane, edge);
 	}

@@ -131,10 +131,22 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 		const int zb = z0 + (bj < z1 - z0 ? bj : 0);
 		const int xb = sx * STRIP_OUT + (right ? STRIP_OUT : 0);   // interface xb - 1/2
 		const Mapped mz = map_coord(zb, nz, g.bc[4], g.bc[5], 4);
-		C cb[4];
+		Mapped mxb[4];
+		bool plain = mz.fixedFace < 0 && !mz.mirror && myOwn.fixedFace < 0 && !myOwn.mirror;
 #pragma unroll
-		for (int k = 0; k < 4; ++k)
-			load_cell3<R, E>(g, src, map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0), myOwn, mz, cb[k]);
+		for (int k = 0; k < 4; ++k) {
+			mxb[k] = map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0);
+			plain = plain && mxb[k].fixedFace < 0 && !mxb[k].mirror;
+		}
+		C cb[4];
+		if (__all(plain)) {
+#pragma unroll
+			for (int k = 0; k < 4; ++k)
+				load_plain<R, E>(g, src, ((long)mz.idx * g.n[1] + myOwn.idx) * g.step1 + mxb[k].idx, cb[k]);
+		} else {
+#pragma unroll
+			for (int k = 0; k < 4; ++k) load_cell3<R, E>(g, src, mxb[k], myOwn, mz, cb[k]);
+		}
 		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_PLANES>(par, lim, slim, cb, dt_dx0, flux_on<E>(g, 0),
 			flux_aov<E>(g, 0), lane, edge);
 	}

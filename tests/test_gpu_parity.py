@@ -126,6 +126,22 @@ def test_strip_edges_2d(rows, mode):
     _check(s, o, 2, TOL64, 8, f'edges-{mode}-{rows}')
 
 
+@pytest.mark.parametrize('fast', [False, True])
+@pytest.mark.parametrize('mode', ['plm', 'flux limiter'])
+def test_strip_edges_2d_mixed_faces(mode, fast):
+    """edge interfaces next to fixed, mirror and freeflow faces: three strips
+    (the last partial), fixed values entering both the edge pass and the rows"""
+    rec = {'plm': dict(usePLM='plm cons', slopeLimiter='minmod'), 'flux limiter': dict(fluxLimiter='van Leer')}[mode]
+    left = [1., .1, 0, 0, 2.5, 0]
+    top = [.5, 0, -.2, 0, 1.2, 0]
+    kw = dict(dim=2, gridSize=(150, 45), integrator='Runge-Kutta 3, TVD', cfl=.3,
+              boundary={'xmin': 'fixed', 'xmax': 'mirror', 'ymin': 'freeflow', 'ymax': 'fixed'},
+              boundaryFixed={'xmin': left, 'ymax': top}, **rec)
+    s, o = _pair(_quad(6), 6, fastMath=fast, **kw)
+    s.setLaunch(7, 0)
+    _check(s, o, 2, TOL64, 6, f'mixed-{mode}-{fast}')
+
+
 @pytest.mark.parametrize('integ', ['forward Euler', 'Runge-Kutta 2', 'Runge-Kutta 2 Heun', 'Runge-Kutta 2 Ralston',
                                    'Runge-Kutta 3', 'Runge-Kutta 4', 'Runge-Kutta 4, 3/8ths rule',
                                    'Runge-Kutta 2, non-TVD', 'Runge-Kutta 4, TVD', 'Runge-Kutta 4, non-TVD'])

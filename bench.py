@@ -103,10 +103,10 @@ def cpu_baseline(seconds=10.0):
                       f'oracle = reference kernel sequence, AoS, unfused, OpenMP; cpu: {cpu}'}
 
 
-def kh_solver(n, real, fast, rows=0):
+def kh_solver(n, real, fast, rows=0, flux='roe'):
     from hydro.solver.hipfvsolver import HipFVSolver
     from hydro.init import euler as init_euler
-    s = HipFVSolver(dim=2, gridSize=(n, n), mins=(-.5, -.5), maxs=(.5, .5), usePLM='plm cons',
+    s = HipFVSolver(dim=2, gridSize=(n, n), mins=(-.5, -.5), maxs=(.5, .5), flux=flux, usePLM='plm cons',
                     slopeLimiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4,
                     boundary='periodic', real=real, fastMath=fast)
     if rows:
@@ -292,6 +292,7 @@ def main():
     ap.add_argument('--size', type=int, default=N)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 KH leg')
+    ap.add_argument('--no-fluxes', action='store_true', help='skip the HLL / Rusanov KH leg')
     ap.add_argument('--no-batch', action='store_true', help='skip the batched leg')
     ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
     ap.add_argument('--no-3d', action='store_true', help='skip the 3D blast leg')
@@ -388,6 +389,18 @@ def main():
                        'rel_l1_vs_fp64': [float(x) for x in e],
                        'note': f'after {args.warmup + args.steps} steps; each form takes its own dt'}
         del s32
+    if not args.no_fluxes and args.real == 'double':
+        # the same workload with the other flux plugins (plm cons/minmod, no flux limiter)
+        out['fluxes'] = {}
+        for flux in ('hll', 'rusanov'):
+            sf = kh_solver(n, args.real, fast, args.rows, flux=flux)
+            steps_f = max(20, args.steps // 5)
+            wallf, evf = timed_steps(sf, args.warmup, steps_f)
+            avg_f = evf / (2 * steps_f)
+            out['fluxes'][flux] = {'value': cells * steps_f / wallf / 1e9, 'unit': 'Gcell-updates/s',
+                                   'steps': steps_f, 'avg_launch_ms': avg_f,
+                                   'roofline_frac': 5 * NQ * 8 / 2 * cells / (avg_f * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            del sf
     if not args.no_batch:
         out['batched'] = batched_leg(args.batch_grids, args.batch_steps)
     if not args.no_3d:

@@ -19,12 +19,15 @@
 
 namespace hydro {
 
-template <bool FAST, int NVEL>
+// FLUX: the interface flux the stage kernels build from this equation
+// (HYDRO_FLUX_ROE, _HLL or _RUSANOV).
+template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE>
 struct EulerT {
 	template <typename R>
 	struct Eqn {
 		using M = Math<R, FAST>;
 		static constexpr bool FASTM = FAST;
+		static constexpr int FLUXID = FLUX;
 		static constexpr int Q = 5;       // numIntStates (euler.lua:14)
 		static constexpr int NSTAT = 0;   // no static fields on the device
 		static constexpr int NW = 5;      // numWaves (euler.lua:13)
@@ -82,6 +85,17 @@ struct EulerT {
 			if (W.P <= p.PMin) return R(0);
 			if (W.rho < p.rhoMin) return R(INFINITY);
 			return M::sqrt_pos(M::div_inv(p.gamma * W.P, W.rho, W.inv));   // P > PMin, rho >= rhoMin
+		}
+
+		// consWaveCodeMinMax (eqn.lua:1134-1146, euler.lua:329-337): the
+		// state's slowest and fastest wave along S, v_n -+ calc_Cs_fromCons
+		// (euler.cl:96-112), v_n = m_n / rho (0 below rhoMin)
+		template <int S>
+		__device__ static __forceinline__ void cons_waves(const Par &p, const R *U, const Prim &W, R &lmin, R &lmax) {
+			R Cs = calc_Cs(p, W);
+			R vn = U[0] < p.rhoMin ? R(0) : M::div_inv(U[1 + S], U[0], W.inv);
+			lmin = vn - Cs;
+			lmax = vn + Cs;
 		}
 
 		// eigen_forInterface (euler.cl:347-430) from the two states and their prims

@@ -20,6 +20,7 @@ template <typename R>
 struct Eqn {
 	using M = Math<R, FAST>;
 	static constexpr bool FASTM = FAST;
+	static constexpr int FLUXID = HYDRO_FLUX_ROE;
 	static constexpr int Q = 8;
 	static constexpr int NSTAT = 4;
 	static constexpr int NW = 8;

@@ -157,7 +157,10 @@ static int validate(const hydro_cfg *c) {
 	for (int i = 0; i < c->dim; ++i)
 		if (c->size[i] < 1) return fail("size[%d] = %d must be >= 1", i, c->size[i]);
 	if (c->eqn != HYDRO_EQN_EULER && c->eqn != HYDRO_EQN_GLM_MAXWELL) return fail("eqn %d not supported", c->eqn);
-	if (c->flux != HYDRO_FLUX_ROE) return fail("flux %d not supported", c->flux);
+	if (c->flux != HYDRO_FLUX_ROE && c->flux != HYDRO_FLUX_HLL && c->flux != HYDRO_FLUX_RUSANOV)
+		return fail("flux %d not supported", c->flux);
+	if (c->flux != HYDRO_FLUX_ROE && c->eqn != HYDRO_EQN_EULER)
+		return fail("the HLL and Rusanov fluxes take the Euler equations");
 	if (c->flux_limiter < 0 || c->flux_limiter >= HYDRO_NUM_LIMITERS) return fail("flux_limiter %d out of range", c->flux_limiter);
 	if (c->slope_limiter < 0 || c->slope_limiter >= HYDRO_NUM_LIMITERS) return fail("slope_limiter %d out of range", c->slope_limiter);
 	if (c->plm != HYDRO_PLM_NONE && c->plm != HYDRO_PLM_CONS) return fail("plm %d not supported", c->plm);
@@ -412,7 +415,9 @@ static StageKey stage_key(const Grid &g) {
 	k.eqn = g.cfg.eqn;
 	k.dim = g.dim;
 	k.plm = g.cfg.plm != HYDRO_PLM_NONE;
-	k.lim = !k.plm && g.cfg.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
+	k.flux = g.cfg.flux;
+	// only the Roe flux uses a flux limiter (fvsolver.lua:63)
+	k.lim = !k.plm && k.flux == HYDRO_FLUX_ROE && g.cfg.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
 	k.limiter = k.plm ? g.cfg.slope_limiter : g.cfg.flux_limiter;
 	k.fast = g.cfg.fast_math != 0;
 	k.nvel = g.nvel;

@@ -19,6 +19,7 @@ struct StageKey {
 	int limiter;    // limiter index (flux limiter if lim, slope limiter if plm)
 	int fast;       // FAST math policy
 	int nvel;       // Euler: active velocity components (dim or 3)
+	int flux;       // HYDRO_FLUX_*
 };
 
 // Limiters with a compile-time form; the rest use the runtime switch.

@@ -182,6 +182,49 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 	for (int j = 0; j < E::Q; ++j) F[j] = Y[j] + I.Favg[j];
 }
 
+// ------------------------------------------------- one interface, no limiter
+// The interface flux of the equation's flux plugin without a flux limiter:
+// Roe (roe.cl:17-163), HLL with the 'Davis direct bounded' estimates
+// (hll.cl:13-73, hll.lua:10), or Rusanov (rusanov.cl:13-39, whose dissipation
+// speed is the right state's: the reference assigns it per side in turn).
+template <typename R, class E, int S>
+__device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell<R, E> &UL, const Cell<R, E> &UR,
+	R dt_dx, int lim, R *F)
+{
+	if constexpr (E::FLUXID == HYDRO_FLUX_ROE) {
+		Iface<R, E> I;
+		iface_prep<R, E, S>(p, UL, UR, I);
+		iface_flux<R, E, S, false, 0>(p, I, nullptr, nullptr, dt_dx, lim, F);
+	} else {
+		using M = typename E::M;
+		auto WL = E::prim(p, UL.v);
+		auto WR = E::prim(p, UR.v);
+		R FL[E::Q], FR[E::Q];
+		E::template flux<S>(p, UL.v, WL, FL);
+		E::template flux<S>(p, UR.v, WR, FR);
+		R lminL, lmaxL, lminR, lmaxR;
+		E::template cons_waves<S>(p, UL.v, WL, lminL, lmaxL);
+		E::template cons_waves<S>(p, UR.v, WR, lminR, lmaxR);
+		if constexpr (E::FLUXID == HYDRO_FLUX_HLL) {
+			auto e = E::eigen(p, UL.v, UR.v, WL, WR);
+			R lam[E::NW];
+			E::template waves<S>(p, e, lam);
+			const R sL = fmin(lminL, lam[0]);
+			const R sR = fmax(lmaxR, lam[E::NW - 1]);
+			const R inv = M::rcp(sR - sL);
+#pragma unroll
+			for (int j = 0; j < E::Q; ++j) {
+				R mid = M::div_inv(sR * FL[j] - sL * FR[j] + sL * sR * (UR.v[j] - UL.v[j]), sR - sL, inv);
+				F[j] = R(0) <= sL ? FL[j] : (sR <= R(0) ? FR[j] : mid);
+			}
+		} else {
+			const R a = fmax(fabs(lminR), fabs(lmaxR));
+#pragma unroll
+			for (int j = 0; j < E::Q; ++j) F[j] = R(.5) * (FL[j] + FR[j] - a * (UR.v[j] - UL.v[j]));
+		}
+	}
+}
+
 // ------------------------------------------------------------ flux guards
 // The reference zeroes a flux where the cell area is <= 1e-7 and the whole
 // derivative where the volume is (fvsolver.lua:105, fvsolver.cl:97). EXACT
@@ -223,11 +266,9 @@ __device__ __forceinline__ void edge_flux(const typename E::Par &par, int lim, i
 		Cell<R, E> L1, R1, L2, R2;
 		plm_cons<R, E, SLIM>(slim, c[0], c[1], c[2], L1, R1);
 		plm_cons<R, E, SLIM>(slim, c[1], c[2], c[3], L2, R2);
-		Iface<R, E> I;
-		iface_prep<R, E, 0>(par, R1, L2, I);
-		iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, F);
+		iface_solve<R, E, 0>(par, R1, L2, dt_dx0, lim, F);
 #pragma unroll
-		for (int j = 0; j < E::NW; ++j) dW[j] = I.dW[j];
+		for (int j = 0; j < E::NW; ++j) dW[j] = R(0);   // kept by the flux limiter only
 	} else if constexpr (LIM) {
 		Iface<R, E> Im, I, Ip;
 		iface_prep<R, E, 0>(par, c[0], c[1], Im);
@@ -237,11 +278,9 @@ __device__ __forceinline__ void edge_flux(const typename E::Par &par, int lim, i
 #pragma unroll
 		for (int j = 0; j < E::NW; ++j) dW[j] = I.dW[j];
 	} else {
-		Iface<R, E> I;
-		iface_prep<R, E, 0>(par, c[1], c[2], I);
-		iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, F);
+		iface_solve<R, E, 0>(par, c[1], c[2], dt_dx0, lim, F);
 #pragma unroll
-		for (int j = 0; j < E::NW; ++j) dW[j] = I.dW[j];
+		for (int j = 0; j < E::NW; ++j) dW[j] = R(0);
 	}
 #pragma unroll
 	for (int j = 0; j < Q; ++j) Fa[j] = fluxOn ? F[j] * aov : R(0);
@@ -325,9 +364,7 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 #pragma unroll
 			for (int f = 0; f < C::NV; ++f) RxL.v[f] = lane == 0 ? Rx.v[f] : RxL.v[f];
 		}
-		Iface<R, E> I;
-		iface_prep<R, E, 0>(par, RxL, Lx, I);
-		iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+		iface_solve<R, E, 0>(par, RxL, Lx, dt_dx0, lim, Fx);
 	} else if constexpr (LIM) {
 		Iface<R, E> I;
 		iface_prep<R, E, 0>(par, Um, U, I);
@@ -347,9 +384,7 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 		}
 		iface_flux<R, E, 0, true, FLIM>(par, I, dWL, dWR, dt_dx0, lim, Fx);
 	} else {
-		Iface<R, E> I;
-		iface_prep<R, E, 0>(par, Um, U, I);
-		iface_flux<R, E, 0, false, 0>(par, I, nullptr, nullptr, dt_dx0, lim, Fx);
+		iface_solve<R, E, 0>(par, Um, U, dt_dx0, lim, Fx);
 	}
 #pragma unroll
 	for (int k = 0; k < Q; ++k) {
@@ -475,9 +510,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				C L1, R1;
 				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
 				if (wantFlux) {
-					Iface<R, E> I;
-					iface_prep<R, E, 1>(par, Ry0, L1, I);
-					iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, F);
+					iface_solve<R, E, 1>(par, Ry0, L1, dt_dx1, lim, F);
 				} else {
 #pragma unroll
 					for (int j = 0; j < Q; ++j) F[j] = R(0);
@@ -495,9 +528,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			} else {
 				// W0 = U(r): interface r+1/2
 				loadRow(r + 1, W1);
-				Iface<R, E> I;
-				iface_prep<R, E, 1>(par, W0, W1, I);
-				iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, F);
+				iface_solve<R, E, 1>(par, W0, W1, dt_dx1, lim, F);
 			}
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {

@@ -193,9 +193,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				C L1, R1;
 				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
 				if (wantFlux) {
-					Iface<R, E> I;
-					iface_prep<R, E, 2>(par, Rz0, L1, I);
-					iface_flux<R, E, 2, false, 0>(par, I, nullptr, nullptr, dt_dx2, lim, F);
+					iface_solve<R, E, 2>(par, Rz0, L1, dt_dx2, lim, F);
 				} else {
 #pragma unroll
 					for (int j = 0; j < Q; ++j) F[j] = R(0);
@@ -215,9 +213,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				I1 = I2;
 			} else {
 				loadPlane(r + 1, W1);
-				Iface<R, E> I;
-				iface_prep<R, E, 2>(par, W0, W1, I);
-				iface_flux<R, E, 2, false, 0>(par, I, nullptr, nullptr, dt_dx2, lim, F);
+				iface_solve<R, E, 2>(par, W0, W1, dt_dx2, lim, F);
 			}
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
@@ -269,13 +265,9 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				__syncthreads();
 				C Rm;
 				lds_get<R, E, NV>(sh.e[w], Rm.v, lane);
-				Iface<R, E> I;
-				iface_prep<R, E, 1>(par, Rm, Ly, I);
-				iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, Fy);
+				iface_solve<R, E, 1>(par, Rm, Ly, dt_dx1, lim, Fy);
 				if (w == W - 1) {
-					Iface<R, E> It;
-					iface_prep<R, E, 1>(par, Ry, Ltop, It);
-					iface_flux<R, E, 1, false, 0>(par, It, nullptr, nullptr, dt_dx1, lim, FyTop);
+					iface_solve<R, E, 1>(par, Ry, Ltop, dt_dx1, lim, FyTop);
 				}
 			} else if constexpr (LIM) {
 				// slot k of sh.e holds dW of interface y0 + k - 3/2
@@ -319,15 +311,11 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			} else {
 				C Um;
 				lds_get<R, E, NV>(sh.u[w + 1], Um.v, lane);
-				Iface<R, E> I;
-				iface_prep<R, E, 1>(par, Um, U, I);
-				iface_flux<R, E, 1, false, 0>(par, I, nullptr, nullptr, dt_dx1, lim, Fy);
+				iface_solve<R, E, 1>(par, Um, U, dt_dx1, lim, Fy);
 				if (w == W - 1) {
 					C a1;
 					lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
-					Iface<R, E> It;
-					iface_prep<R, E, 1>(par, U, a1, It);
-					iface_flux<R, E, 1, false, 0>(par, It, nullptr, nullptr, dt_dx1, lim, FyTop);
+					iface_solve<R, E, 1>(par, U, a1, dt_dx1, lim, FyTop);
 				}
 			}
 			R FaY[Q];

@@ -1,7 +1,9 @@
-"""Flux plugins (hydro/flux/*.lua): the Roe flux of this path."""
+"""Flux plugins (hydro/flux/*.lua): Roe, HLL and Rusanov."""
 from .roe import Roe
+from .hll import HLL
+from .rusanov import Rusanov
 
-FLUXES = {'roe': Roe}
+FLUXES = {'roe': Roe, 'hll': HLL, 'rusanov': Rusanov}
 
 
 def get(name_or_obj):

@@ -28,8 +28,10 @@ extern "C" {
  * phi, psi, rhoCharge, sigma, sqrt_1_eps, sqrt_1_mu; 8 integrated, 8 waves). */
 enum { HYDRO_EQN_EULER = 0, HYDRO_EQN_GLM_MAXWELL = 1 };
 
-/* Flux plugins: hydro/flux/roe.lua. */
-enum { HYDRO_FLUX_ROE = 0 };
+/* Flux plugins: hydro/flux/roe.lua, hll.lua ('Davis direct bounded' wave
+ * estimates, hll.lua:10) and rusanov.lua. HLL and Rusanov take the Euler
+ * equations and ignore the flux limiter (only Roe uses one, fvsolver.lua:63). */
+enum { HYDRO_FLUX_ROE = 0, HYDRO_FLUX_HLL = 1, HYDRO_FLUX_RUSANOV = 2 };
 
 /* Limiter table in the order of HydroCLApp.limiters (hydro/app.lua:614-636),
  * 0-based: the reference's index 1 ('donor cell') is 0 here, and a flux limiter

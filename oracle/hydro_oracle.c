@@ -6,6 +6,7 @@
  *   Euler eqn        hydro/eqn/euler.cl:1-173, 271-291, 343-542, 697-717
  *   GLM-Maxwell eqn  hydro/eqn/glm-maxwell.cl:96-380, glm-maxwell.lua:181-245
  *   Roe flux         hydro/flux/roe.cl:17-163
+ *   HLL, Rusanov     hydro/flux/hll.cl:13-73 (hll.lua:10), rusanov.cl:13-39
  *   plm cons         hydro/solver/plm.cl:27-91, 976-997
  *   calcFlux         hydro/solver/fvsolver.lua:55-199
  *   calcDerivFromFlux hydro/solver/fvsolver.cl:1-125
@@ -539,6 +540,66 @@ void oracle_roe_flux(const oracle_params *p, int eqn, int flux_limiter, double d
 	for (int j = 0; j < q; ++j) F[j] += .5 * (FL[j] + FR[j]);
 }
 
+/* ------------------------------------------------------------ HLL, Rusanov */
+
+static real euler_cs_from_cons(const oracle_params *p, const real *U) {
+	/* calc_Cs_fromCons (euler.cl:96-112) */
+	real P = euler_calc_P(p, U);
+	if (P <= p->P_min) return 0.;
+	if (U[0] < p->rho_min) return INFINITY;
+	return sqrt(p->gamma * P / U[0]);
+}
+
+static void euler_cons_wave_min_max(const oracle_params *p, const real *U, int s, real *lmin, real *lmax) {
+	/* consWaveCodeMinMax (eqn.lua:1134-1146) with euler.lua:329-337:
+	 * v_n = m_n / rho (0 below rhoMin), v_n -+ Cs */
+	real Cs = euler_cs_from_cons(p, U);
+	real v_n = U[0] < p->rho_min ? 0. : U[1 + s] / U[0];
+	*lmin = v_n - Cs;
+	*lmax = v_n + Cs;
+}
+
+void oracle_hll_flux(const oracle_params *p, const double *UL, const double *UR, int s, double *F) {
+	/* hll.cl:13-73, 'Davis direct bounded' (hll.lua:10); Euler */
+	real eig[10];
+	oracle_euler_eigen_for_interface(p, UL, UR, eig);
+	real lambdas[5];
+	oracle_euler_waves(p, eig, s, lambdas);
+	real lambdaIntMin = lambdas[0], lambdaIntMax = lambdas[4];
+	real lambdaLMin, lambdaLMax, lambdaRMin, lambdaRMax;
+	euler_cons_wave_min_max(p, UL, s, &lambdaLMin, &lambdaLMax);
+	euler_cons_wave_min_max(p, UR, s, &lambdaRMin, &lambdaRMax);
+	real sL = fmin(lambdaLMin, lambdaIntMin);
+	real sR = fmax(lambdaRMax, lambdaIntMax);
+	real FL[6], FR[6];
+	oracle_euler_flux_from_cons(p, UL, s, FL);
+	oracle_euler_flux_from_cons(p, UR, s, FR);
+	for (int j = 0; j < 5; ++j) {
+		if (0 <= sL) F[j] = FL[j];
+		else if (sR <= 0) F[j] = FR[j];
+		else
+			/* also the NaN case, where the reference leaves the flux buffer's
+			 * earlier contents */
+			F[j] = (sR * FL[j] - sL * FR[j] + sL * sR * (UR[j] - UL[j])) / (sR - sL);
+	}
+	F[5] = 0;
+}
+
+void oracle_rusanov_flux(const oracle_params *p, const double *UL, const double *UR, int s, double *F) {
+	/* rusanov.cl:13-39: lambdaMax is assigned for the left state, then for the
+	 * right state, so the right state's value is used */
+	real lambdaMax, lmin, lmax;
+	euler_cons_wave_min_max(p, UL, s, &lmin, &lmax);
+	lambdaMax = fmax(fabs(lmin), fabs(lmax));
+	euler_cons_wave_min_max(p, UR, s, &lmin, &lmax);
+	lambdaMax = fmax(fabs(lmin), fabs(lmax));
+	real FL[6], FR[6];
+	oracle_euler_flux_from_cons(p, UL, s, FL);
+	oracle_euler_flux_from_cons(p, UR, s, FR);
+	for (int j = 0; j < 5; ++j) F[j] = .5 * (FL[j] + FR[j] - lambdaMax * (UR[j] - UL[j]));
+	F[5] = 0;
+}
+
 void oracle_plm_cons(int lim, int nint, int S, const double *UL, const double *U, const double *UR,
 	double *L, double *R)
 {
@@ -646,6 +707,12 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 	if (cfg->plm && cfg->flux_limiter != HYDRO_LIMITER_DONOR_CELL) {
 		set_err("are you sure you want to use flux and slope limiters at the same time?");
 		return NULL;
+	}
+	if (cfg->flux != HYDRO_FLUX_ROE && (cfg->flux != HYDRO_FLUX_HLL && cfg->flux != HYDRO_FLUX_RUSANOV)) {
+		set_err("bad flux"); return NULL;
+	}
+	if (cfg->flux != HYDRO_FLUX_ROE && cfg->eqn != HYDRO_EQN_EULER) {
+		set_err("HLL and Rusanov take the Euler equations"); return NULL;
 	}
 	oracle_solver *s = (oracle_solver *)calloc(1, sizeof *s);
 	s->cfg = *cfg;
@@ -850,8 +917,11 @@ static void calc_flux(oracle_solver *s, const real *U, real dt) {
 			const real *limargs[4] = {
 				U + (indexL - s->step[side]) * S, U + indexL * S,
 				U + index * S, U + (index + s->step[side]) * S};
-			oracle_roe_flux(&s->p, s->cfg.eqn, lim, dt_dx, UL, UR,
-				lim != HYDRO_LIMITER_DONOR_CELL ? limargs : NULL, side, F);
+			if (s->cfg.flux == HYDRO_FLUX_HLL) oracle_hll_flux(&s->p, UL, UR, side, F);
+			else if (s->cfg.flux == HYDRO_FLUX_RUSANOV) oracle_rusanov_flux(&s->p, UL, UR, side, F);
+			else
+				oracle_roe_flux(&s->p, s->cfg.eqn, lim, dt_dx, UL, UR,
+					lim != HYDRO_LIMITER_DONOR_CELL ? limargs : NULL, side, F);
 		}
 	});
 }

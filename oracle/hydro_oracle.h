@@ -63,6 +63,11 @@ void oracle_glm_left(const oracle_params *p, const double *eig, const double *X,
 void oracle_glm_right(const oracle_params *p, const double *eig, const double *X, int side, double *Y);
 void oracle_glm_flux_from_cons(const oracle_params *p, const double *U, int side, double *F);
 void oracle_glm_waves(const oracle_params *p, const double *eig, int side, double *lambda);
+/* HLL ('Davis direct bounded', hll.cl:13-73) and Rusanov (rusanov.cl:13-39)
+ * fluxes of the Euler equations for one interface; F gets 6 reals. */
+void oracle_hll_flux(const oracle_params *p, const double *UL, const double *UR, int side, double *F);
+void oracle_rusanov_flux(const oracle_params *p, const double *UL, const double *UR, int side, double *F);
+
 /* Roe flux for one interface (roe.cl:17-163). lim_args may be NULL when no
  * flux limiter: otherwise {UL_L, UR_L, UL_R, UR_R} pointers. */
 void oracle_roe_flux(const oracle_params *p, int eqn, int flux_limiter, double dt_dx,

@@ -30,7 +30,7 @@ def _pair(init, nsteps, **kw):
 
 
 def _oracle_kw(kw):
-    m = {'dim': 'dim', 'gridSize': 'size', 'mins': 'mins', 'maxs': 'maxs', 'usePLM': 'plm', 'eqn': 'eqn',
+    m = {'dim': 'dim', 'gridSize': 'size', 'mins': 'mins', 'maxs': 'maxs', 'usePLM': 'plm', 'eqn': 'eqn', 'flux': 'flux',
          'slopeLimiter': 'slope_limiter', 'fluxLimiter': 'flux_limiter', 'integrator': 'integrator',
          'cfl': 'cfl', 'boundary': 'boundary', 'fixedDT': 'fixed_dt', 'boundaryFixed': 'boundary_fixed'}
     out = {m[k]: v for k, v in kw.items() if k in m}

@@ -215,6 +215,88 @@ def test_roe_flux_limiter_lax_wendroff_and_donor():
 
 
 # ------------------------------------------------------------------ PLM
+def _hll(UL, UR, side):
+    F = np.zeros(6)
+    lib().oracle_hll_flux(ctypes.byref(P), _ptr(UL), _ptr(UR), side, _ptr(F))
+    return F[:5]
+
+
+def _rusanov(UL, UR, side):
+    F = np.zeros(6)
+    lib().oracle_rusanov_flux(ctypes.byref(P), _ptr(UL), _ptr(UR), side, _ptr(F))
+    return F[:5]
+
+
+def _textbook_speeds(U, side, gamma=1.4):
+    rho, m, E = U[0], U[1:4], U[4]
+    v = m / rho
+    p = (gamma - 1) * (E - .5 * rho * v @ v)
+    return v[side], np.sqrt(gamma * p / rho), (E + p) / rho
+
+
+def test_hll_rusanov_consistency_and_upwinding():
+    for U in _random_states(6, seed=31):
+        for side in range(3):
+            assert np.allclose(_hll(U, U, side), _flux(U, side), rtol=1e-14, atol=1e-14)
+            assert np.allclose(_rusanov(U, U, side), _flux(U, side), rtol=1e-14, atol=1e-14)
+    W = lambda rho, v, p: np.array([rho, *v, p, 0.])
+    for sgn in (1, -1):
+        UL = np.zeros(6)
+        UR = np.zeros(6)
+        lib().oracle_euler_cons_from_prim(ctypes.byref(P), _ptr(W(1., (sgn * 5., .1, 0), 1.)), _ptr(UL))
+        lib().oracle_euler_cons_from_prim(ctypes.byref(P), _ptr(W(.8, (sgn * 5.2, -.1, 0), .9)), _ptr(UR))
+        want = _flux(UL, 0) if sgn > 0 else _flux(UR, 0)
+        assert np.array_equal(_hll(UL, UR, 0), want)
+
+
+def test_hll_intermediate_state_rankine_hugoniot():
+    """For sL < 0 < sR the HLL flux is the flux of the one intermediate state
+    U* = (sR UR - sL UL - (FR - FL)) / (sR - sL) across either wave:
+    F = FL + sL (U* - UL) = FR + sR (U* - UR), with the Davis bounds
+    sL = min(vL - cL, v~ - c~), sR = max(vR + cR, v~ + c~) from the Roe
+    averages written out here"""
+    n = 0
+    for i, (UL, UR) in enumerate(zip(_random_states(12, seed=32), _random_states(12, seed=33))):
+        for side in range(3):
+            vL, cL, HL = _textbook_speeds(UL, side)
+            vR, cR, HR = _textbook_speeds(UR, side)
+            sl, sr = np.sqrt(UL[0]), np.sqrt(UR[0])
+            vRoe = (sl * UL[1:4] / UL[0] + sr * UR[1:4] / UR[0]) / (sl + sr)
+            HRoe = (sl * HL + sr * HR) / (sl + sr)
+            cRoe = np.sqrt(.4 * (HRoe - .5 * vRoe @ vRoe))
+            sL = min(vL - cL, vRoe[side] - cRoe)
+            sR = max(vR + cR, vRoe[side] + cRoe)
+            if not (sL < 0 < sR):
+                continue
+            n += 1
+            FL, FR = _flux(UL, side), _flux(UR, side)
+            Us = (sR * UR[:5] - sL * UL[:5] - (FR - FL)) / (sR - sL)
+            F = _hll(UL, UR, side)
+            assert np.allclose(F, FL + sL * (Us - UL[:5]), rtol=1e-12, atol=1e-12)
+            assert np.allclose(F, FR + sR * (Us - UR[:5]), rtol=1e-12, atol=1e-12)
+    assert n >= 10
+
+
+def test_rusanov_dissipation_from_right_state():
+    """F = (FL + FR)/2 - a (UR - UL)/2 with a = |vR| + cR: the reference sets a
+    for the left state and then for the right one, which stays, even when the
+    left state is much faster"""
+    W = lambda rho, v, p: np.array([rho, *v, p, 0.])
+    UL = np.zeros(6)
+    UR = np.zeros(6)
+    lib().oracle_euler_cons_from_prim(ctypes.byref(P), _ptr(W(1., (4., .3, 0), 1.)), _ptr(UL))
+    lib().oracle_euler_cons_from_prim(ctypes.byref(P), _ptr(W(.5, (-.2, .1, 0), .4)), _ptr(UR))
+    for side in range(2):
+        vR, cR, _ = _textbook_speeds(UR, side)
+        a = abs(vR) + cR
+        want = .5 * (_flux(UL, side) + _flux(UR, side)) - .5 * a * (UR[:5] - UL[:5])
+        assert np.allclose(_rusanov(UL, UR, side), want, rtol=1e-13, atol=1e-13)
+        vL, cL, _ = _textbook_speeds(UL, side)
+        aL = abs(vL) + cL
+        if side == 0:
+            assert aL > 2 * a   # the left state's speed would have given another flux
+
+
 def _plm(Um, U, Up, limiter='minmod'):
     L = np.zeros(6)
     R = np.zeros(6)
@@ -475,6 +557,30 @@ def test_sod_against_exact_riemann():
     ex = _exact_sod(x, o.t)
     m = (np.abs(x - .05) < .05)
     assert np.allclose(rho[m], ex[m], rtol=.02)
+
+
+@pytest.mark.parametrize('flux,bound', [('hll', 2e-2), ('rusanov', 2.5e-2)])
+def test_sod_hll_rusanov_against_exact_riemann(flux, bound):
+    """the same Sod run with the HLL and Rusanov fluxes (first order, no
+    limiter: more diffusive than Roe + superbee); the error against the exact
+    solution falls when the grid is refined to the same time"""
+    left = [1., 0, 0, 0, 2.5, 0]
+    right = [.125, 0, 0, 0, .25, 0]
+    errs = []
+    for n, steps in ((256, 100), (512, 200)):
+        cfg = make_cfg(dim=1, size=n, flux=flux, cfl=.5,
+                       boundary={'xmin': 'fixed', 'xmax': 'fixed'}, boundary_fixed={'xmin': left, 'xmax': right})
+        o = pyoracle.Oracle(cfg)
+        o.set_U(init_euler.to_ghosted(cfg, init_euler.sod(cfg)))
+        o.boundary()
+        o.constrain_U()
+        o.update(steps)
+        x = (np.arange(n) + .5) / n * 2 - 1
+        rho = o.interior()[0, 0, :, 0]
+        errs.append(np.abs(rho - _exact_sod(x, o.t)).mean() * 2)
+    print(flux, 'L1(rho) at 256, 512:', errs)
+    assert errs[0] < bound
+    assert errs[1] < .8 * errs[0]
 
 
 # ------------------------------------------------------------ GLM-Maxwell

@@ -32,8 +32,11 @@ def test_euler_prim_cons_roundtrip():
 def test_flux_plugin():
     f = flux.get('roe')
     assert f.name == 'roe' and f.usesFluxLimiter and not f.useEntropyFluxFix
+    h, r = flux.get('hll'), flux.get('rusanov')
+    assert h.name == 'hll' and not h.usesFluxLimiter and h.hllCalcWaveMethod == 'Davis direct bounded'
+    assert r.name == 'rusanov' and not r.usesFluxLimiter
     with pytest.raises(ValueError):
-        flux.get('hll')
+        flux.get('hllc')
 
 
 def _R(alphas, betas, z):

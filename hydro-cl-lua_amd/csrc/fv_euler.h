@@ -20,7 +20,7 @@
 namespace hydro {
 
 // FLUX: the interface flux the stage kernels build from this equation
-// (HYDRO_FLUX_ROE, _HLL or _RUSANOV).
+// (HYDRO_FLUX_ROE, _HLL, _RUSANOV or _HLLC).
 template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE>
 struct EulerT {
 	template <typename R>

@@ -185,8 +185,11 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 // ------------------------------------------------- one interface, no limiter
 // The interface flux of the equation's flux plugin without a flux limiter:
 // Roe (roe.cl:17-163), HLL with the 'Davis direct bounded' estimates
-// (hll.cl:13-73, hll.lua:10), or Rusanov (rusanov.cl:13-39, whose dissipation
-// speed is the right state's: the reference assigns it per side in turn).
+// (hll.cl:13-73, hll.lua:10), Rusanov (rusanov.cl:13-39, whose dissipation
+// speed is the right state's: the reference assigns it per side in turn), or
+// HLLC (euler-hllc.cl:13-240, hllcMethod 2, HLL's estimates). The cases of
+// HLL and HLLC are selects over all candidate fluxes, in the reference's
+// expression order each.
 template <typename R, class E, int S>
 __device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell<R, E> &UL, const Cell<R, E> &UR,
 	R dt_dx, int lim, R *F)
@@ -216,6 +219,48 @@ __device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell
 			for (int j = 0; j < E::Q; ++j) {
 				R mid = M::div_inv(sR * FL[j] - sL * FR[j] + sL * sR * (UR.v[j] - UL.v[j]), sR - sL, inv);
 				F[j] = R(0) <= sL ? FL[j] : (sR <= R(0) ? FR[j] : mid);
+			}
+		} else if constexpr (E::FLUXID == HYDRO_FLUX_HLLC) {
+			auto e = E::eigen(p, UL.v, UR.v, WL, WR);
+			R lam[E::NW];
+			E::template waves<S>(p, e, lam);
+			const R sL = fmin(lminL, lam[0]);
+			const R sR = fmax(lmaxR, lam[E::NW - 1]);
+			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;   // normal basis (coord.lua:2393-2408)
+			const R vnL = WL.v[A], vnR = WR.v[A];
+			const R sStar = M::div(WR.rho * vnR * (sR - vnR) - WL.rho * vnL * (sL - vnL) + WL.P - WR.P,
+				WR.rho * (sR - vnR) - WL.rho * (sL - vnL));
+			const R PLR = R(.5) * (WL.P + WR.P + WL.rho * (sL - vnL) * (sStar - vnL)
+				+ WR.rho * (sR - vnR) * (sStar - vnR));
+			const R dL = sL - sStar, dR = sR - sStar;
+			const R iL = M::rcp(dL), iR = M::rcp(dR);
+			R FsL[E::Q], FsR[E::Q];
+			FsL[0] = M::div_inv((sL * UL.v[0] - FL[0]) * sStar, dL, iL);
+			FsL[1 + A] = M::div_inv((sL * UL.v[1 + A] - FL[1 + A]) * sStar + sL * PLR, dL, iL);
+			FsL[1 + B] = M::div_inv(sStar * (sL * UL.v[1 + B] - FL[1 + B]), dL, iL);
+			FsL[1 + C] = M::div_inv(sStar * (sL * UL.v[1 + C] - FL[1 + C]), dL, iL);
+			FsL[4] = M::div_inv(sStar * (sL * UL.v[4] - FL[4]) + sL * PLR * sStar, dL, iL);
+			FsR[0] = M::div_inv(sStar * (sR * UR.v[0] - FR[0]), dR, iR);
+			FsR[1 + A] = M::div_inv(sStar * (sR * UR.v[1 + A] - FR[1 + A]) + sR * PLR, dR, iR);
+			FsR[1 + B] = M::div_inv(sStar * (sR * UR.v[1 + B] - FR[1 + B]), dR, iR);
+			FsR[1 + C] = M::div_inv(sStar * (sR * UR.v[1 + C] - FR[1 + C]), dR, iR);
+			FsR[4] = M::div_inv(sStar * (sR * UR.v[4] - FR[4]) + sR * PLR * sStar, dR, iR);
+			const bool useFL = R(0) <= sL;
+			const bool useL = sL <= R(0) && R(0) <= sStar;
+			const bool useR = sStar <= R(0) && R(0) <= sR;
+			const bool useFR = sR <= R(0);
+#pragma unroll
+			for (int j = 0; j < E::Q; ++j) F[j] = useFL ? FL[j] : useL ? FsL[j] : useR ? FsR[j] : FR[j];
+			// HLL where sStar is not a number (and the NaN case): a wave-uniform
+			// branch, as no lane of a finite state takes it
+			const bool hllCase = !(useFL || useL || useR || useFR);
+			if (__any(hllCase)) {
+				const R dLR = sR - sL, iLR = M::rcp(dLR);
+#pragma unroll
+				for (int j = 0; j < E::Q; ++j) {
+					R hll = M::div_inv(sR * FL[j] - sL * FR[j] + sL * sR * (UR.v[j] - UL.v[j]), dLR, iLR);
+					F[j] = hllCase ? hll : F[j];
+				}
 			}
 		} else {
 			const R a = fmax(fabs(lminR), fabs(lmaxR));

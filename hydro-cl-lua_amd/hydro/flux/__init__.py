@@ -1,9 +1,10 @@
-"""Flux plugins (hydro/flux/*.lua): Roe, HLL and Rusanov."""
+"""Flux plugins (hydro/flux/*.lua): Roe, HLL, Rusanov and HLLC."""
 from .roe import Roe
 from .hll import HLL
 from .rusanov import Rusanov
+from .hllc import EulerHLLC
 
-FLUXES = {'roe': Roe, 'hll': HLL, 'rusanov': Rusanov}
+FLUXES = {'roe': Roe, 'hll': HLL, 'rusanov': Rusanov, 'euler-hllc': EulerHLLC}
 
 
 def get(name_or_obj):

@@ -7,6 +7,7 @@
  *   GLM-Maxwell eqn  hydro/eqn/glm-maxwell.cl:96-380, glm-maxwell.lua:181-245
  *   Roe flux         hydro/flux/roe.cl:17-163
  *   HLL, Rusanov     hydro/flux/hll.cl:13-73 (hll.lua:10), rusanov.cl:13-39
+ *   HLLC             hydro/flux/euler-hllc.cl:13-240 (hllcMethod 2)
  *   plm cons         hydro/solver/plm.cl:27-91, 976-997
  *   calcFlux         hydro/solver/fvsolver.lua:55-199
  *   calcDerivFromFlux hydro/solver/fvsolver.cl:1-125
@@ -585,6 +586,65 @@ void oracle_hll_flux(const oracle_params *p, const double *UL, const double *UR,
 	F[5] = 0;
 }
 
+void oracle_hllc_flux(const oracle_params *p, const double *UL, const double *UR, int s, double *F) {
+	/* euler-hllc.cl:13-240: hllcMethod 2 (euler-hllc.lua:18) with the 'Davis
+	 * direct bounded' estimates it inherits from hll.lua:10. Momenta in the
+	 * normal basis (n, t1, t2) = (e_s, e_(s+1)%3, e_(s+2)%3) (coord.lua:2393-2408) */
+	real WL[6], WR[6];
+	oracle_euler_prim_from_cons(p, UL, WL);
+	oracle_euler_prim_from_cons(p, UR, WR);
+	real eig[10];
+	oracle_euler_eigen_for_interface(p, UL, UR, eig);
+	real lambdas[5];
+	oracle_euler_waves(p, eig, s, lambdas);
+	real lambdaIntMin = lambdas[0], lambdaIntMax = lambdas[4];
+	real lambdaLMin, lambdaLMax, lambdaRMin, lambdaRMax;
+	euler_cons_wave_min_max(p, UL, s, &lambdaLMin, &lambdaLMax);
+	euler_cons_wave_min_max(p, UR, s, &lambdaRMin, &lambdaRMax);
+	real sL = fmin(lambdaLMin, lambdaIntMin);
+	real sR = fmax(lambdaRMax, lambdaIntMax);
+	int a = s, b = (s + 1) % 3, c = (s + 2) % 3;
+	real vnL[3] = {WL[1 + a], WL[1 + b], WL[1 + c]};
+	real vnR[3] = {WR[1 + a], WR[1 + b], WR[1 + c]};
+	real sStar = (WR[0] * vnR[0] * (sR - vnR[0]) - WL[0] * vnL[0] * (sL - vnL[0]) + WL[4] - WR[4])
+		/ (WR[0] * (sR - vnR[0]) - WL[0] * (sL - vnL[0]));
+	real FL[6], FR[6];
+	oracle_euler_flux_from_cons(p, UL, s, FL);
+	oracle_euler_flux_from_cons(p, UR, s, FR);
+	real fn[3];
+	if (0 <= sL) {
+		for (int j = 0; j < 5; ++j) F[j] = FL[j];
+	} else if (sL <= 0. && 0. <= sStar) {
+		real PLR = .5 * (WL[4] + WR[4] + WL[0] * (sL - vnL[0]) * (sStar - vnL[0])
+			+ WR[0] * (sR - vnR[0]) * (sStar - vnR[0]));
+		F[0] = (sL * UL[0] - FL[0]) * sStar / (sL - sStar);
+		real Umn[3] = {UL[1 + a], UL[1 + b], UL[1 + c]}, Fmn[3] = {FL[1 + a], FL[1 + b], FL[1 + c]};
+		fn[0] = ((sL * Umn[0] - Fmn[0]) * sStar + sL * PLR) / (sL - sStar);
+		fn[1] = sStar * (sL * Umn[1] - Fmn[1]) / (sL - sStar);
+		fn[2] = sStar * (sL * Umn[2] - Fmn[2]) / (sL - sStar);
+		F[1 + a] = fn[0]; F[1 + b] = fn[1]; F[1 + c] = fn[2];
+		F[4] = (sStar * (sL * UL[4] - FL[4]) + sL * PLR * sStar) / (sL - sStar);
+	} else if (sStar <= 0. && 0. <= sR) {
+		real PLR = .5 * (WL[4] + WR[4] + WL[0] * (sL - vnL[0]) * (sStar - vnL[0])
+			+ WR[0] * (sR - vnR[0]) * (sStar - vnR[0]));
+		F[0] = sStar * (sR * UR[0] - FR[0]) / (sR - sStar);
+		real Umn[3] = {UR[1 + a], UR[1 + b], UR[1 + c]}, Fmn[3] = {FR[1 + a], FR[1 + b], FR[1 + c]};
+		fn[0] = (sStar * (sR * Umn[0] - Fmn[0]) + sR * PLR) / (sR - sStar);
+		fn[1] = sStar * (sR * Umn[1] - Fmn[1]) / (sR - sStar);
+		fn[2] = sStar * (sR * Umn[2] - Fmn[2]) / (sR - sStar);
+		F[1 + a] = fn[0]; F[1 + b] = fn[1]; F[1 + c] = fn[2];
+		F[4] = (sStar * (sR * UR[4] - FR[4]) + sR * PLR * sStar) / (sR - sStar);
+	} else if (sR <= 0) {
+		for (int j = 0; j < 5; ++j) F[j] = FR[j];
+	} else {
+		/* sL <= 0 <= sR with sStar not a number: HLL (also the NaN case,
+		 * where the reference leaves the buffer's earlier contents) */
+		for (int j = 0; j < 5; ++j)
+			F[j] = (sR * FL[j] - sL * FR[j] + sL * sR * (UR[j] - UL[j])) / (sR - sL);
+	}
+	F[5] = 0;
+}
+
 void oracle_rusanov_flux(const oracle_params *p, const double *UL, const double *UR, int s, double *F) {
 	/* rusanov.cl:13-39: lambdaMax is assigned for the left state, then for the
 	 * right state, so the right state's value is used */
@@ -708,11 +768,12 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 		set_err("are you sure you want to use flux and slope limiters at the same time?");
 		return NULL;
 	}
-	if (cfg->flux != HYDRO_FLUX_ROE && (cfg->flux != HYDRO_FLUX_HLL && cfg->flux != HYDRO_FLUX_RUSANOV)) {
+	if (cfg->flux != HYDRO_FLUX_ROE && cfg->flux != HYDRO_FLUX_HLL && cfg->flux != HYDRO_FLUX_RUSANOV
+		&& cfg->flux != HYDRO_FLUX_HLLC) {
 		set_err("bad flux"); return NULL;
 	}
 	if (cfg->flux != HYDRO_FLUX_ROE && cfg->eqn != HYDRO_EQN_EULER) {
-		set_err("HLL and Rusanov take the Euler equations"); return NULL;
+		set_err("HLL, Rusanov and HLLC take the Euler equations"); return NULL;
 	}
 	oracle_solver *s = (oracle_solver *)calloc(1, sizeof *s);
 	s->cfg = *cfg;
@@ -919,6 +980,7 @@ static void calc_flux(oracle_solver *s, const real *U, real dt) {
 				U + index * S, U + (index + s->step[side]) * S};
 			if (s->cfg.flux == HYDRO_FLUX_HLL) oracle_hll_flux(&s->p, UL, UR, side, F);
 			else if (s->cfg.flux == HYDRO_FLUX_RUSANOV) oracle_rusanov_flux(&s->p, UL, UR, side, F);
+			else if (s->cfg.flux == HYDRO_FLUX_HLLC) oracle_hllc_flux(&s->p, UL, UR, side, F);
 			else
 				oracle_roe_flux(&s->p, s->cfg.eqn, lim, dt_dx, UL, UR,
 					lim != HYDRO_LIMITER_DONOR_CELL ? limargs : NULL, side, F);

@@ -67,6 +67,8 @@ void oracle_glm_waves(const oracle_params *p, const double *eig, int side, doubl
  * fluxes of the Euler equations for one interface; F gets 6 reals. */
 void oracle_hll_flux(const oracle_params *p, const double *UL, const double *UR, int side, double *F);
 void oracle_rusanov_flux(const oracle_params *p, const double *UL, const double *UR, int side, double *F);
+/* HLLC (euler-hllc.cl:13-240, hllcMethod 2) of the Euler equations */
+void oracle_hllc_flux(const oracle_params *p, const double *UL, const double *UR, int side, double *F);
 
 /* Roe flux for one interface (roe.cl:17-163). lim_args may be NULL when no
  * flux limiter: otherwise {UL_L, UR_L, UL_R, UR_R} pointers. */

@@ -227,6 +227,45 @@ def _rusanov(UL, UR, side):
     return F[:5]
 
 
+def _hllc(UL, UR, side):
+    F = np.zeros(6)
+    lib().oracle_hllc_flux(ctypes.byref(P), _ptr(UL), _ptr(UR), side, _ptr(F))
+    return F[:5]
+
+
+def _cons(rho, v, p):
+    U = np.zeros(6)
+    lib().oracle_euler_cons_from_prim(ctypes.byref(P), _ptr(np.array([rho, *v, p, 0.])), _ptr(U))
+    return U
+
+
+def test_hllc_consistency_upwinding_and_contacts():
+    """HLLC: F(U,U) = F(U); supersonic states upwind exactly; an isolated
+    contact or shear wave (same pressure and normal velocity on both sides) is
+    resolved exactly, the flux being the upwind state's physical flux, where
+    HLL would diffuse it"""
+    for U in _random_states(6, seed=34):
+        for side in range(3):
+            assert np.allclose(_hllc(U, U, side), _flux(U, side), rtol=1e-13, atol=1e-13)
+    for sgn in (1, -1):
+        UL = _cons(1., (sgn * 5., .1, 0), 1.)
+        UR = _cons(.8, (sgn * 5.2, -.1, 0), .9)
+        want = _flux(UL, 0) if sgn > 0 else _flux(UR, 0)
+        assert np.array_equal(_hllc(UL, UR, 0), want)
+    for side in range(3):
+        for u in (0., .3, -.4):
+            vL = np.zeros(3); vR = np.zeros(3)
+            vL[side] = vR[side] = u
+            vL[(side + 1) % 3], vR[(side + 1) % 3] = .2, -.5      # shear
+            UL, UR = _cons(1.7, vL, .8), _cons(.3, vR, .8)         # contact
+            up = UL if u >= 0 else UR
+            F = _hllc(UL, UR, side)
+            assert np.allclose(F, _flux(up, side), rtol=1e-13, atol=1e-13), (side, u, F, _flux(up, side))
+            if u == 0:
+                assert abs(F[0]) < 1e-15
+                assert not np.allclose(_hll(UL, UR, side)[0], 0, atol=1e-3)   # HLL does diffuse it
+
+
 def _textbook_speeds(U, side, gamma=1.4):
     rho, m, E = U[0], U[1:4], U[4]
     v = m / rho
@@ -559,7 +598,7 @@ def test_sod_against_exact_riemann():
     assert np.allclose(rho[m], ex[m], rtol=.02)
 
 
-@pytest.mark.parametrize('flux,bound', [('hll', 2e-2), ('rusanov', 2.5e-2)])
+@pytest.mark.parametrize('flux,bound', [('hll', 2e-2), ('rusanov', 2.5e-2), ('euler-hllc', 2e-2)])
 def test_sod_hll_rusanov_against_exact_riemann(flux, bound):
     """the same Sod run with the HLL and Rusanov fluxes (first order, no
     limiter: more diffusive than Roe + superbee); the error against the exact

@@ -292,7 +292,7 @@ def main():
     ap.add_argument('--size', type=int, default=N)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 KH leg')
-    ap.add_argument('--no-fluxes', action='store_true', help='skip the HLL / Rusanov / HLLC KH leg')
+    ap.add_argument('--no-fluxes', action='store_true', help='skip the HLL / Rusanov / HLLC / Marquina KH leg')
     ap.add_argument('--no-batch', action='store_true', help='skip the batched leg')
     ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
     ap.add_argument('--no-3d', action='store_true', help='skip the 3D blast leg')
@@ -392,7 +392,7 @@ def main():
     if not args.no_fluxes and args.real == 'double':
         # the same workload with the other flux plugins (plm cons/minmod, no flux limiter)
         out['fluxes'] = {}
-        for flux in ('hll', 'rusanov', 'euler-hllc'):
+        for flux in ('hll', 'rusanov', 'euler-hllc', 'marquina'):
             sf = kh_solver(n, args.real, fast, args.rows, flux=flux)
             steps_f = max(20, args.steps // 5)
             wallf, evf = timed_steps(sf, args.warmup, steps_f)

@@ -20,7 +20,7 @@
 namespace hydro {
 
 // FLUX: the interface flux the stage kernels build from this equation
-// (HYDRO_FLUX_ROE, _HLL, _RUSANOV or _HLLC).
+// (HYDRO_FLUX_ROE, _HLL, _RUSANOV, _HLLC or _MARQUINA).
 template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE>
 struct EulerT {
 	template <typename R>
@@ -153,6 +153,22 @@ struct EulerT {
 				e.hTotal = hTotal;
 				e.Cs = M::sqrt_pos(p.g1 * h);        // h >= rhoEpsilon
 			}
+			e.vSq = vSq;
+			return e;
+		}
+
+		// eigen_forCell (euler.cl:315-341): one cell's own state, no guards
+		__device__ static __forceinline__ Eig eigen_cell(const Par &p, const R *U, const Prim &W) {
+			Eig e;
+			e.rho = W.rho;
+#pragma unroll
+			for (int i = 0; i < 3; ++i) e.v[i] = W.v[i];
+			R vSq = lenSq3(W.v);
+			R eKin = R(.5) * vSq;
+			R hTotal = M::div_inv(W.P + U[4], W.rho, W.inv);   // calc_hTotal = (P + ETotal) / rho
+			R CsSq = p.g1 * (hTotal - eKin);
+			e.hTotal = hTotal;
+			e.Cs = M::sqrt_(CsSq);
 			e.vSq = vSq;
 			return e;
 		}

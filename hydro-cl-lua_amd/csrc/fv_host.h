@@ -157,11 +157,10 @@ static int validate(const hydro_cfg *c) {
 	for (int i = 0; i < c->dim; ++i)
 		if (c->size[i] < 1) return fail("size[%d] = %d must be >= 1", i, c->size[i]);
 	if (c->eqn != HYDRO_EQN_EULER && c->eqn != HYDRO_EQN_GLM_MAXWELL) return fail("eqn %d not supported", c->eqn);
-	if (c->flux != HYDRO_FLUX_ROE && c->flux != HYDRO_FLUX_HLL && c->flux != HYDRO_FLUX_RUSANOV
-		&& c->flux != HYDRO_FLUX_HLLC)
+	if (c->flux < HYDRO_FLUX_ROE || c->flux > HYDRO_FLUX_MARQUINA)
 		return fail("flux %d not supported", c->flux);
 	if (c->flux != HYDRO_FLUX_ROE && c->eqn != HYDRO_EQN_EULER)
-		return fail("the HLL, Rusanov and HLLC fluxes take the Euler equations");
+		return fail("the HLL, Rusanov, HLLC and Marquina fluxes take the Euler equations");
 	if (c->flux_limiter < 0 || c->flux_limiter >= HYDRO_NUM_LIMITERS) return fail("flux_limiter %d out of range", c->flux_limiter);
 	if (c->slope_limiter < 0 || c->slope_limiter >= HYDRO_NUM_LIMITERS) return fail("slope_limiter %d out of range", c->slope_limiter);
 	if (c->plm != HYDRO_PLM_NONE && c->plm != HYDRO_PLM_CONS) return fail("plm %d not supported", c->plm);

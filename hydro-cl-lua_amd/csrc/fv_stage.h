@@ -187,9 +187,9 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 // Roe (roe.cl:17-163), HLL with the 'Davis direct bounded' estimates
 // (hll.cl:13-73, hll.lua:10), Rusanov (rusanov.cl:13-39, whose dissipation
 // speed is the right state's: the reference assigns it per side in turn), or
-// HLLC (euler-hllc.cl:13-240, hllcMethod 2, HLL's estimates). The cases of
-// HLL and HLLC are selects over all candidate fluxes, in the reference's
-// expression order each.
+// HLLC (euler-hllc.cl:13-240, hllcMethod 2, HLL's estimates), or Marquina
+// (marquina.cl:5-93). The cases of HLL and HLLC are selects over all
+// candidate fluxes, in the reference's expression order each.
 template <typename R, class E, int S>
 __device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell<R, E> &UL, const Cell<R, E> &UR,
 	R dt_dx, int lim, R *F)
@@ -220,6 +220,36 @@ __device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell
 				R mid = M::div_inv(sR * FL[j] - sL * FR[j] + sL * sR * (UR.v[j] - UL.v[j]), sR - sL, inv);
 				F[j] = R(0) <= sL ? FL[j] : (sR <= R(0) ? FR[j] : mid);
 			}
+		} else if constexpr (E::FLUXID == HYDRO_FLUX_MARQUINA) {
+			// each side's state and flux in its own eigenbasis; per wave,
+			// upwind when both sides' speeds share a sign, else the local
+			// Lax-Friedrichs split with the larger |lambda|
+			auto eL = E::eigen_cell(p, UL.v, WL);
+			auto eR = E::eigen_cell(p, UR.v, WR);
+			R omegaL[E::NW], omegaR[E::NW], phiL[E::NW], phiR[E::NW];
+			E::template left<S>(p, eL, UL.v, omegaL);
+			E::template left<S>(p, eR, UR.v, omegaR);
+			E::template left<S>(p, eL, FL, phiL);
+			E::template left<S>(p, eR, FR, phiR);
+			R lL[E::NW], lR[E::NW];
+			E::template waves<S>(p, eL, lL);
+			E::template waves<S>(p, eR, lR);
+			R pp[E::NW], pm[E::NW];
+#pragma unroll
+			for (int k = 0; k < E::NW; ++k) {
+				const bool same = lL[k] * lR[k] > R(0);
+				const bool right = lL[k] > R(0);
+				const R alpha = fmax(fabs(lL[k]), fabs(lR[k]));
+				const R splitP = (phiL[k] + alpha * omegaL[k]) * R(.5);
+				const R splitM = (phiR[k] - alpha * omegaR[k]) * R(.5);
+				pp[k] = same ? (right ? phiL[k] : R(0)) : splitP;
+				pm[k] = same ? (right ? R(0) : phiR[k]) : splitM;
+			}
+			R YL[E::Q], YR[E::Q];
+			E::template right<S>(p, eL, pp, YL);
+			E::template right<S>(p, eR, pm, YR);
+#pragma unroll
+			for (int j = 0; j < E::Q; ++j) F[j] = YL[j] + YR[j];
 		} else if constexpr (E::FLUXID == HYDRO_FLUX_HLLC) {
 			auto e = E::eigen(p, UL.v, UR.v, WL, WR);
 			R lam[E::NW];

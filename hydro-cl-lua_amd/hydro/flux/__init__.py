@@ -1,10 +1,11 @@
-"""Flux plugins (hydro/flux/*.lua): Roe, HLL, Rusanov and HLLC."""
+"""Flux plugins (hydro/flux/*.lua): Roe, HLL, Rusanov, HLLC and Marquina."""
 from .roe import Roe
 from .hll import HLL
 from .rusanov import Rusanov
 from .hllc import EulerHLLC
+from .marquina import Marquina
 
-FLUXES = {'roe': Roe, 'hll': HLL, 'rusanov': Rusanov, 'euler-hllc': EulerHLLC}
+FLUXES = {'roe': Roe, 'hll': HLL, 'rusanov': Rusanov, 'euler-hllc': EulerHLLC, 'marquina': Marquina}
 
 
 def get(name_or_obj):

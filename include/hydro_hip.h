@@ -29,11 +29,12 @@ extern "C" {
 enum { HYDRO_EQN_EULER = 0, HYDRO_EQN_GLM_MAXWELL = 1 };
 
 /* Flux plugins: hydro/flux/roe.lua, hll.lua ('Davis direct bounded' wave
- * estimates, hll.lua:10), rusanov.lua and euler-hllc.lua (hllcMethod 2,
- * euler-hllc.lua:18, with HLL's wave estimates). HLL, Rusanov and HLLC take
- * the Euler equations and ignore the flux limiter (only Roe uses one,
+ * estimates, hll.lua:10), rusanov.lua, euler-hllc.lua (hllcMethod 2,
+ * euler-hllc.lua:18, with HLL's wave estimates) and marquina.lua. All but Roe
+ * take the Euler equations and ignore the flux limiter (only Roe uses one,
  * fvsolver.lua:63). */
-enum { HYDRO_FLUX_ROE = 0, HYDRO_FLUX_HLL = 1, HYDRO_FLUX_RUSANOV = 2, HYDRO_FLUX_HLLC = 3 };
+enum { HYDRO_FLUX_ROE = 0, HYDRO_FLUX_HLL = 1, HYDRO_FLUX_RUSANOV = 2, HYDRO_FLUX_HLLC = 3,
+	HYDRO_FLUX_MARQUINA = 4 };
 
 /* Limiter table in the order of HydroCLApp.limiters (hydro/app.lua:614-636),
  * 0-based: the reference's index 1 ('donor cell') is 0 here, and a flux limiter

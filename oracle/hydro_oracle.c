@@ -8,6 +8,7 @@
  *   Roe flux         hydro/flux/roe.cl:17-163
  *   HLL, Rusanov     hydro/flux/hll.cl:13-73 (hll.lua:10), rusanov.cl:13-39
  *   HLLC             hydro/flux/euler-hllc.cl:13-240 (hllcMethod 2)
+ *   Marquina         hydro/flux/marquina.cl:5-93, eigen_forCell euler.cl:315-341
  *   plm cons         hydro/solver/plm.cl:27-91, 976-997
  *   calcFlux         hydro/solver/fvsolver.lua:55-199
  *   calcDerivFromFlux hydro/solver/fvsolver.cl:1-125
@@ -645,6 +646,61 @@ void oracle_hllc_flux(const oracle_params *p, const double *UL, const double *UR
 	F[5] = 0;
 }
 
+void oracle_euler_eigen_for_cell(const oracle_params *p, const double *U, double *e) {
+	/* eigen_forCell (euler.cl:315-341): the cell's own state, no averaging */
+	real W[6];
+	oracle_euler_prim_from_cons(p, U, W);
+	real vSq = dot3(W + 1, W + 1);
+	real eKin = .5 * vSq;
+	real hTotal = (W[4] + U[4]) / W[0];   /* calc_hTotal = calc_HTotal / rho */
+	real CsSq = (p->gamma - 1.) * (hTotal - eKin);
+	e[E_RHO] = W[0];
+	e[1] = W[1]; e[2] = W[2]; e[3] = W[3];
+	e[E_HT] = hTotal;
+	e[E_CS] = sqrt(CsSq);
+	e[E_VSQ] = vSq;
+	e[E_VL] = W[1]; e[E_VL + 1] = W[2]; e[E_VL + 2] = W[3];
+}
+
+void oracle_marquina_flux(const oracle_params *p, const double *UL, const double *UR, int s, double *F) {
+	/* marquina.cl:5-93 */
+	real FL[6], FR[6];
+	oracle_euler_flux_from_cons(p, UL, s, FL);
+	oracle_euler_flux_from_cons(p, UR, s, FR);
+	real eL[10], eR[10];
+	oracle_euler_eigen_for_cell(p, UL, eL);
+	oracle_euler_eigen_for_cell(p, UR, eR);
+	real omegaL[5], omegaR[5], phiL[5], phiR[5];
+	oracle_euler_left(p, eL, UL, s, omegaL);
+	oracle_euler_left(p, eR, UR, s, omegaR);
+	oracle_euler_left(p, eL, FL, s, phiL);
+	oracle_euler_left(p, eR, FR, s, phiR);
+	real lL[5], lR[5];
+	oracle_euler_waves(p, eL, s, lL);
+	oracle_euler_waves(p, eR, s, lR);
+	real pp[5], pm[5];
+	for (int k = 0; k < 5; ++k) {
+		if (lL[k] * lR[k] > 0) {
+			if (lL[k] > 0) {
+				pp[k] = phiL[k];
+				pm[k] = 0;
+			} else {
+				pp[k] = 0;
+				pm[k] = phiR[k];
+			}
+		} else {
+			real alpha = fmax(fabs(lL[k]), fabs(lR[k]));
+			pp[k] = (phiL[k] + alpha * omegaL[k]) * .5;
+			pm[k] = (phiR[k] - alpha * omegaR[k]) * .5;
+		}
+	}
+	real YL[6], YR[6];
+	oracle_euler_right(p, eL, pp, s, YL);
+	oracle_euler_right(p, eR, pm, s, YR);
+	for (int k = 0; k < 5; ++k) F[k] = YL[k] + YR[k];
+	F[5] = 0;
+}
+
 void oracle_rusanov_flux(const oracle_params *p, const double *UL, const double *UR, int s, double *F) {
 	/* rusanov.cl:13-39: lambdaMax is assigned for the left state, then for the
 	 * right state, so the right state's value is used */
@@ -768,12 +824,11 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 		set_err("are you sure you want to use flux and slope limiters at the same time?");
 		return NULL;
 	}
-	if (cfg->flux != HYDRO_FLUX_ROE && cfg->flux != HYDRO_FLUX_HLL && cfg->flux != HYDRO_FLUX_RUSANOV
-		&& cfg->flux != HYDRO_FLUX_HLLC) {
+	if (cfg->flux < HYDRO_FLUX_ROE || cfg->flux > HYDRO_FLUX_MARQUINA) {
 		set_err("bad flux"); return NULL;
 	}
 	if (cfg->flux != HYDRO_FLUX_ROE && cfg->eqn != HYDRO_EQN_EULER) {
-		set_err("HLL, Rusanov and HLLC take the Euler equations"); return NULL;
+		set_err("HLL, Rusanov, HLLC and Marquina take the Euler equations"); return NULL;
 	}
 	oracle_solver *s = (oracle_solver *)calloc(1, sizeof *s);
 	s->cfg = *cfg;
@@ -981,6 +1036,7 @@ static void calc_flux(oracle_solver *s, const real *U, real dt) {
 			if (s->cfg.flux == HYDRO_FLUX_HLL) oracle_hll_flux(&s->p, UL, UR, side, F);
 			else if (s->cfg.flux == HYDRO_FLUX_RUSANOV) oracle_rusanov_flux(&s->p, UL, UR, side, F);
 			else if (s->cfg.flux == HYDRO_FLUX_HLLC) oracle_hllc_flux(&s->p, UL, UR, side, F);
+			else if (s->cfg.flux == HYDRO_FLUX_MARQUINA) oracle_marquina_flux(&s->p, UL, UR, side, F);
 			else
 				oracle_roe_flux(&s->p, s->cfg.eqn, lim, dt_dx, UL, UR,
 					lim != HYDRO_LIMITER_DONOR_CELL ? limargs : NULL, side, F);

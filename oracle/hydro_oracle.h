@@ -69,6 +69,9 @@ void oracle_hll_flux(const oracle_params *p, const double *UL, const double *UR,
 void oracle_rusanov_flux(const oracle_params *p, const double *UL, const double *UR, int side, double *F);
 /* HLLC (euler-hllc.cl:13-240, hllcMethod 2) of the Euler equations */
 void oracle_hllc_flux(const oracle_params *p, const double *UL, const double *UR, int side, double *F);
+/* eigen_forCell (euler.cl:315-341) and the Marquina flux (marquina.cl:5-93) */
+void oracle_euler_eigen_for_cell(const oracle_params *p, const double *U, double *eig);
+void oracle_marquina_flux(const oracle_params *p, const double *UL, const double *UR, int side, double *F);
 
 /* Roe flux for one interface (roe.cl:17-163). lim_args may be NULL when no
  * flux limiter: otherwise {UL_L, UR_L, UL_R, UR_R} pointers. */

@@ -1,6 +1,6 @@
-"""GPU: the HLL, Rusanov and HLLC fluxes (hydro/flux/hll.cl, rusanov.cl,
-euler-hllc.cl) through the
-C ABI against the CPU oracle, to the same bars as the Roe flux: FP64 within
+"""GPU: the HLL, Rusanov, HLLC and Marquina fluxes (hydro/flux/hll.cl,
+rusanov.cl, euler-hllc.cl, marquina.cl) through the C ABI against the CPU
+oracle, to the same bars as the Roe flux: FP64 within
 1e-12 relative L1 per conserved variable (IEEE and fast math), FP32 bulk
 within 1e-5 of the FP64 oracle; 1D, 2D (several strips, every boundary kind)
 and 3D; the batched form equal to the plain one; the flux limiter ignored as
@@ -13,7 +13,7 @@ from test_gpu_parity import _pair, _check, _quad, _sod, TOL64, TOL32
 
 pytestmark = pytest.mark.gpu
 
-FLUXES = ['hll', 'rusanov', 'euler-hllc']
+FLUXES = ['hll', 'rusanov', 'euler-hllc', 'marquina']
 
 
 @pytest.mark.parametrize('fast', [False, True])

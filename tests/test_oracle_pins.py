@@ -266,6 +266,39 @@ def test_hllc_consistency_upwinding_and_contacts():
                 assert not np.allclose(_hll(UL, UR, side)[0], 0, atol=1e-3)   # HLL does diffuse it
 
 
+def _marquina(UL, UR, side):
+    F = np.zeros(6)
+    lib().oracle_marquina_flux(ctypes.byref(P), _ptr(UL), _ptr(UR), side, _ptr(F))
+    return F[:5]
+
+
+def test_marquina_consistency_upwinding_and_mirror_symmetry():
+    """Marquina: F(U,U) = F(U) and supersonic states upwind (each up to the
+    round-off of R L = I); the flux is symmetric under reflection of the
+    normal axis: swapping the sides and flipping the normal momenta flips the
+    flux, F(M UR, M UL) = -M F(UL, UR), for states whose waves change sign
+    across the interface as well (the Lax-Friedrichs split) - a swapped split
+    term, sign or side's eigenbasis breaks it"""
+    for U in _random_states(6, seed=35):
+        for side in range(3):
+            assert np.allclose(_marquina(U, U, side), _flux(U, side), rtol=1e-12, atol=1e-12)
+    for sgn in (1, -1):
+        UL = _cons(1., (sgn * 5., .1, 0), 1.)
+        UR = _cons(.8, (sgn * 5.2, -.1, 0), .9)
+        want = _flux(UL, 0) if sgn > 0 else _flux(UR, 0)
+        assert np.allclose(_marquina(UL, UR, 0), want, rtol=1e-12, atol=1e-12)
+    n = 0
+    for UL, UR in zip(_random_states(10, seed=36), _random_states(10, seed=37)):
+        for side in range(3):
+            Mx = np.ones(5)
+            Mx[1 + side] = -1
+            F = _marquina(UL, UR, side)
+            Fm = _marquina(np.r_[UR[:5] * Mx, 0], np.r_[UL[:5] * Mx, 0], side)
+            assert np.allclose(Fm, -Mx * F, rtol=1e-12, atol=1e-12)
+            n += not np.allclose(F, _roe(UL, UR, side), rtol=1e-6)
+    assert n > 20   # a flux of its own, not Roe's
+
+
 def _textbook_speeds(U, side, gamma=1.4):
     rho, m, E = U[0], U[1:4], U[4]
     v = m / rho
@@ -598,7 +631,8 @@ def test_sod_against_exact_riemann():
     assert np.allclose(rho[m], ex[m], rtol=.02)
 
 
-@pytest.mark.parametrize('flux,bound', [('hll', 2e-2), ('rusanov', 2.5e-2), ('euler-hllc', 2e-2)])
+@pytest.mark.parametrize('flux,bound', [('hll', 2e-2), ('rusanov', 2.5e-2), ('euler-hllc', 2e-2),
+                                        ('marquina', 2e-2)])
 def test_sod_hll_rusanov_against_exact_riemann(flux, bound):
     """the same Sod run with the HLL and Rusanov fluxes (first order, no
     limiter: more diffusive than Roe + superbee); the error against the exact

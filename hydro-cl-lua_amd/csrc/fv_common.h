@@ -103,6 +103,8 @@ struct StageArgs {
 	// equation parameters in the stencil type and in double (for the FP64
 	// accumulation, constrainU and dt work of the FP32 form)
 	R gamma, rhoMin, PMin, divPhi, divPsi;
+	R roeFixFactor;           // Roe, Euler: Harten's entropy fix when roeFix (roe.cl:83-106)
+	int roeFix;
 	double gammaD, rhoMinD, PMinD, divPhiD, divPsiD;
 	int limiter;              // flux or slope limiter index (runtime switch form)
 	int xcdRemap;

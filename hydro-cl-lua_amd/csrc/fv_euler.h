@@ -39,6 +39,8 @@ struct EulerT {
 
 		struct Par {
 			R gamma, g1, rhoMin, PMin;
+			R fixFactor;   // Roe: Harten's entropy fix when fix
+			bool fix;
 		};
 		struct Prim {
 			R rho, v[3], P, inv;   // inv = 1 / rho
@@ -52,6 +54,8 @@ struct EulerT {
 			p.gamma = a.gamma;
 			p.g1 = a.gamma - R(1);
 			p.rhoMin = a.rhoMin;
+			p.fixFactor = a.roeFixFactor;
+			p.fix = a.roeFix != 0;
 			p.PMin = a.PMin;
 			return p;
 		}

@@ -57,6 +57,7 @@ static inline void cfg_default(hydro_cfg *c) {
 	c->slope_limiter = HYDRO_LIMITER_DONOR_CELL;
 	c->integrator = HYDRO_INT_FORWARD_EULER;
 	c->cfl = .5;
+	c->roe_entropy_fix_factor = .2;
 	for (int i = 0; i < 6; ++i) c->boundary[i] = HYDRO_BC_FREEFLOW;
 	c->precision = HYDRO_REAL_F64;
 	c->gamma = 7. / 5.;
@@ -161,6 +162,10 @@ static int validate(const hydro_cfg *c) {
 		return fail("flux %d not supported", c->flux);
 	if (c->flux != HYDRO_FLUX_ROE && c->eqn != HYDRO_EQN_EULER)
 		return fail("the HLL, Rusanov, HLLC and Marquina fluxes take the Euler equations");
+	if (c->roe_entropy_fix && (c->flux != HYDRO_FLUX_ROE || c->eqn != HYDRO_EQN_EULER))
+		return fail("the entropy fix belongs to the Roe flux of the Euler equations (roe.cl:83-106)");
+	if (c->roe_entropy_fix && !(c->roe_entropy_fix_factor > 0))
+		return fail("roe_entropy_fix_factor must be > 0");
 	if (c->flux_limiter < 0 || c->flux_limiter >= HYDRO_NUM_LIMITERS) return fail("flux_limiter %d out of range", c->flux_limiter);
 	if (c->slope_limiter < 0 || c->slope_limiter >= HYDRO_NUM_LIMITERS) return fail("slope_limiter %d out of range", c->slope_limiter);
 	if (c->plm != HYDRO_PLM_NONE && c->plm != HYDRO_PLM_CONS) return fail("plm %d not supported", c->plm);
@@ -402,6 +407,8 @@ static void stage_args(const Grid &g, int s, StageArgs<R> &a) {
 	a.PMinD = c.P_min;
 	a.divPhiD = c.div_phi_wavespeed;
 	a.divPsiD = c.div_psi_wavespeed;
+	a.roeFix = c.roe_entropy_fix != 0;
+	a.roeFixFactor = R(c.roe_entropy_fix_factor);
 	a.limiter = c.plm ? c.slope_limiter : c.flux_limiter;
 	a.resetSlot = -1;
 	a.reduceSlot = -1;

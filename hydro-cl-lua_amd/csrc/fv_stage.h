@@ -158,6 +158,17 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 #pragma unroll
 	for (int j = 0; j < E::NW; ++j) {
 		R lambda = I.lambda[j];
+		if constexpr (!E::SOURCE) {
+			// Harten's entropy fix (roe.cl:83-106): |lambda| below dws is
+			// replaced by (lambda^2/dws + dws)/2 with OpenCL's sign (0 at 0)
+			if (p.fix) {
+				const R dws = p.fixFactor * I.e.Cs;
+				if (fabs(lambda) < dws) {
+					const R sgn = lambda > R(0) ? R(1) : (lambda < R(0) ? R(-1) : lambda);
+					lambda = sgn * R(.5) * (E::M::div(lambda * lambda, dws) + dws);
+				}
+			}
+		}
 		R sgnLambda = lambda >= R(0) ? R(1) : R(-1);
 		if constexpr (LIM && E::FASTM) {
 			// -lambda/2 (dW sgn + phi(r) dW (lambda dt/dx - sgn)), phi(r) dW without r

@@ -329,7 +329,8 @@ static int same_kernel_config(const hydro_cfg &a, const hydro_cfg &b) {
 		&& a.precision == b.precision && a.fast_math == b.fast_math && a.gamma == b.gamma
 		&& a.rho_min == b.rho_min && a.P_min == b.P_min && a.div_phi_wavespeed == b.div_phi_wavespeed
 		&& a.div_psi_wavespeed == b.div_psi_wavespeed && a.use_fixed_dt == b.use_fixed_dt
-		&& (!a.use_fixed_dt || a.fixed_dt == b.fixed_dt);
+		&& (!a.use_fixed_dt || a.fixed_dt == b.fixed_dt) && a.roe_entropy_fix == b.roe_entropy_fix
+		&& (!a.roe_entropy_fix || a.roe_entropy_fix_factor == b.roe_entropy_fix_factor);
 }
 
 static void batch_drop_graph(hydro_batch *b) {

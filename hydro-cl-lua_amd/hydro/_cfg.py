@@ -61,6 +61,8 @@ class HydroCfg(ctypes.Structure):
         ('div_phi_wavespeed', ctypes.c_double),
         ('div_psi_wavespeed', ctypes.c_double),
         ('fast_math', ctypes.c_int),
+        ('roe_entropy_fix', ctypes.c_int),
+        ('roe_entropy_fix_factor', ctypes.c_double),
     ]
 
 
@@ -80,7 +82,8 @@ def make_cfg(dim, size, mins=None, maxs=None, eqn='euler', flux='roe',
              integrator='forward Euler', cfl=0.5, fixed_dt=None,
              boundary='freeflow', boundary_fixed=None, precision='double',
              gamma=7 / 5, rho_min=1e-7, P_min=1e-7,
-             div_phi_wavespeed=1.0, div_psi_wavespeed=1.0, fast_math=False):
+             div_phi_wavespeed=1.0, div_psi_wavespeed=1.0, fast_math=False,
+             use_entropy_flux_fix=False, entropy_flux_fix_limiting_factor=0.2):
     """Builds a HydroCfg from reference-style arguments (names or indices).
 
     `boundary` is a name for all faces or a dict face -> name, like the
@@ -119,6 +122,9 @@ def make_cfg(dim, size, mins=None, maxs=None, eqn='euler', flux='roe',
     c.div_phi_wavespeed = float(div_phi_wavespeed)
     c.div_psi_wavespeed = float(div_psi_wavespeed)
     c.fast_math = 1 if fast_math else 0
+    # Roe: args.useEntropyFluxFix / entropyFluxFixLimitingFactor (roe.lua:10-16)
+    c.roe_entropy_fix = 1 if use_entropy_flux_fix else 0
+    c.roe_entropy_fix_factor = float(entropy_flux_fix_limiting_factor)
     return c
 
 

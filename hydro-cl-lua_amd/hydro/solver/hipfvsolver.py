@@ -52,12 +52,13 @@ class HipFVSolver:
     def __init__(self, dim, gridSize, eqn='euler', flux='roe', integrator='forward Euler',
                  fluxLimiter='donor cell', usePLM=None, slopeLimiter='minmod', cfl=.5,
                  fixedDT=None, mins=None, maxs=None, boundary='freeflow', boundaryFixed=None,
-                 initCond=None, initCondArgs=None, real='double', eqnArgs=None, fastMath=False):
+                 initCond=None, initCondArgs=None, real='double', eqnArgs=None, fastMath=False,
+                 fluxArgs=None):
         # plugins by name or as objects (hydro.eqn / hydro.flux / hydro.int)
         self.eqn = eqn_plugins.get(eqn)
         eqnArgs = dict(getattr(self.eqn, 'guiVars', {}), **(eqnArgs or {}))
         self.eqn.guiVars = eqnArgs
-        self.flux = flux_plugins.get(flux)
+        self.flux = flux_plugins.get(flux, **(fluxArgs or {}))
         self.integrator = int_plugins.get(integrator)
         self.cfg = make_cfg(dim=dim, size=gridSize, mins=mins, maxs=maxs, eqn=self.eqn.name.replace('-', '_'),
                             flux=self.flux.name, flux_limiter=fluxLimiter, plm=usePLM, slope_limiter=slopeLimiter,
@@ -67,7 +68,9 @@ class HipFVSolver:
                             rho_min=eqnArgs.get('rhoMin', 1e-7), P_min=eqnArgs.get('PMin', 1e-7),
                             div_phi_wavespeed=eqnArgs.get('divPhiWavespeed', 1.),
                             div_psi_wavespeed=eqnArgs.get('divPsiWavespeed', 1.),
-                            fast_math=fastMath)
+                            fast_math=fastMath,
+                            use_entropy_flux_fix=getattr(self.flux, 'useEntropyFluxFix', False),
+                            entropy_flux_fix_limiting_factor=getattr(self.flux, 'entropyFluxFixLimitingFactor', .2))
         self.dim = int(dim)
         self.dtype = np.float64 if real == 'double' else np.float32
         self.shape = cfg_ghost_shape(self.cfg)

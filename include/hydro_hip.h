@@ -123,6 +123,9 @@ typedef struct hydro_cfg {
 	int fast_math;           /* 0: IEEE division and sqrt, the reference's arithmetic;
 	                          * 1: faithfully rounded hardware sequences (FP64: rcp +
 	                          * Newton + residual correction; FP32: v_rcp / v_sqrt) */
+	int roe_entropy_fix;     /* Roe flux, Euler: Harten's entropy fix (args.useEntropyFluxFix,
+	                          * roe.lua:10-16, roe.cl:83-106; default off) */
+	double roe_entropy_fix_factor; /* args.entropyFluxFixLimitingFactor (roe.lua:11, 0.2) */
 } hydro_cfg;
 
 /* Reference defaults: mins -1, maxs 1 (gridsolver.lua:80-81), gamma 7/5,

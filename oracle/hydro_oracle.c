@@ -523,6 +523,12 @@ void oracle_roe_flux(const oracle_params *p, int eqn, int flux_limiter, double d
 	}
 	for (int j = 0; j < nw; ++j) {
 		real lambda = lambdas[j];
+		if (p->roe_fix && eqn == HYDRO_EQN_EULER) {
+			/* roe.cl:83-106 (OpenCL sign: 0 at 0) */
+			real dws = p->roe_fix_factor * eig[E_CS];
+			real sgn = lambda > 0 ? 1. : lambda < 0 ? -1. : lambda;
+			if (fabs(lambda) < dws) lambda = sgn * .5 * (lambda * lambda / dws + dws);
+		}
 		fluxEig[j] = 0.;
 		real sgnLambda = lambda >= 0 ? 1 : -1;
 		if (useLim) {
@@ -834,6 +840,7 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 	s->cfg = *cfg;
 	s->p.gamma = cfg->gamma; s->p.rho_min = cfg->rho_min; s->p.P_min = cfg->P_min;
 	s->p.div_phi = cfg->div_phi_wavespeed; s->p.div_psi = cfg->div_psi_wavespeed;
+	s->p.roe_fix = cfg->roe_entropy_fix; s->p.roe_fix_factor = cfg->roe_entropy_fix_factor;
 	s->dim = cfg->dim;
 	s->S = eqn_S(cfg->eqn);
 	s->q = eqn_q(cfg->eqn);

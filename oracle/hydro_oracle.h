@@ -23,6 +23,8 @@ typedef struct oracle_solver oracle_solver;
 /* equation parameters the per-cell functions read from solver_t */
 typedef struct oracle_params {
 	double gamma, rho_min, P_min, div_phi, div_psi;
+	int roe_fix;                 /* Roe, Euler: Harten's entropy fix (roe.cl:83-106) */
+	double roe_fix_factor;
 } oracle_params;
 
 oracle_solver *oracle_create(const hydro_cfg *cfg);

@@ -26,7 +26,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 class OracleParams(ctypes.Structure):
     _fields_ = [('gamma', ctypes.c_double), ('rho_min', ctypes.c_double),
                 ('P_min', ctypes.c_double), ('div_phi', ctypes.c_double),
-                ('div_psi', ctypes.c_double)]
+                ('div_psi', ctypes.c_double), ('roe_fix', ctypes.c_int),
+                ('roe_fix_factor', ctypes.c_double)]
 
 
 def lib():
@@ -85,8 +86,8 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
-def params(gamma=1.4, rho_min=1e-7, P_min=1e-7, div_phi=1.0, div_psi=1.0):
-    return OracleParams(gamma, rho_min, P_min, div_phi, div_psi)
+def params(gamma=1.4, rho_min=1e-7, P_min=1e-7, div_phi=1.0, div_psi=1.0, roe_fix=0, roe_fix_factor=.2):
+    return OracleParams(gamma, rho_min, P_min, div_phi, div_psi, roe_fix, roe_fix_factor)
 
 
 class Oracle:

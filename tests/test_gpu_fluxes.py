@@ -124,3 +124,38 @@ def test_glm_rejects_hll():
     with pytest.raises(HydroError):
         HipFVSolver(dim=2, gridSize=(32, 32), eqn='glm_maxwell', flux='hll', integrator='Runge-Kutta 2, TVD',
                     cfl=.4, boundary='periodic')
+
+
+@pytest.mark.parametrize('fast', [False, True])
+@pytest.mark.parametrize('rec', ['plm', 'flux limiter'])
+def test_roe_entropy_fix(rec, fast):
+    """Roe with Harten's entropy fix (roe.cl:83-106, args.useEntropyFluxFix) on
+    transonic quadrant states: the oracle's bar, and results that differ from
+    the unfixed flux"""
+    import pyoracle
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as ie
+    kw = dict(usePLM='plm cons', slopeLimiter='minmod') if rec == 'plm' else dict(fluxLimiter='superbee')
+    okw = dict(plm='plm cons', slope_limiter='minmod') if rec == 'plm' else dict(flux_limiter='superbee')
+    outs = []
+    for fix in (True, False):
+        s = HipFVSolver(dim=2, gridSize=(130, 40), integrator='Runge-Kutta 2, TVD', cfl=.3, boundary='freeflow',
+                        fastMath=fast, fluxArgs={'useEntropyFluxFix': fix, 'entropyFluxFixLimitingFactor': .3}, **kw)
+        rng = np.random.default_rng(11)
+        states = ie.random_quadrant_states(rng)
+        U0 = ie.to_ghosted(s.cfg, ie.quadrant(s.cfg, states))
+        s.resetState(U0)
+        s.update(6)
+        outs.append(s.UBufObj.toCPU())
+        if fix:
+            o = pyoracle.Oracle(pyoracle.make_cfg(dim=2, size=(130, 40), integrator='Runge-Kutta 2, TVD', cfl=.3,
+                                                  boundary='freeflow', precision='double', use_entropy_flux_fix=True,
+                                                  entropy_flux_fix_limiting_factor=.3, **okw))
+            o.set_U(U0)
+            o.boundary()
+            o.constrain_U()
+            o.update(6)
+            err = rel_l1(o.interior(outs[0])[..., :5], o.interior()[..., :5])
+            print(f'roe-fix-{rec}-{fast}: rel L1 {err}')
+            assert np.all(err <= TOL64), err
+    assert not np.array_equal(outs[0], outs[1])

@@ -369,6 +369,45 @@ def test_rusanov_dissipation_from_right_state():
             assert aL > 2 * a   # the left state's speed would have given another flux
 
 
+def _roe_fix(UL, UR, side, factor=.2):
+    Pf = params(gamma=1.4, roe_fix=1, roe_fix_factor=factor)
+    F = np.zeros(6)
+    lib().oracle_roe_flux(ctypes.byref(Pf), 0, 0, 0., _ptr(UL), _ptr(UR), None, side, _ptr(F))
+    return F[:5]
+
+
+def test_roe_entropy_fix_harten_hyman():
+    """Harten's fix (roe.cl:83-106) changes only waves with |lambda| below
+    delta = factor * c~, to (lambda^2 + delta^2) / (2 delta) in magnitude:
+    F_fix - F = -1/2 R diag(|lambda|_fix - |lambda|) L (UR - UL), with the
+    textbook formula here and R, L, lambda from the pinned transforms"""
+    # no slow wave: the fix leaves the flux bit for bit
+    c = np.sqrt(1.4)
+    UL, UR = _cons(1., (.5 * c, .1, 0), 1.), _cons(.9, (.45 * c, .05, 0), .95)
+    assert np.array_equal(_roe_fix(UL, UR, 0), _roe(UL, UR, 0))
+    # transonic: v - c near 0 at the Roe state
+    n = 0
+    for side in range(3):
+        for dv in (-.05, .03):
+            vL = np.zeros(3); vR = np.zeros(3)
+            vL[side], vR[side] = c * (1 + dv), c * (1 - dv)
+            vL[(side + 1) % 3] = .2
+            UL, UR = _cons(1., vL, 1.), _cons(1.05, vR, 1.02)
+            e = _eig(UL, UR)
+            lam = np.zeros(6)
+            lib().oracle_euler_waves(ctypes.byref(P), _ptr(e), side, _ptr(lam))
+            lam = lam[:5]
+            delta = .2 * e[5]
+            fixed = np.where(np.abs(lam) < delta, (lam ** 2 + delta ** 2) / (2 * delta), np.abs(lam))
+            assert (np.abs(lam) < delta).sum() == 1
+            Lm = _mat(lib().oracle_euler_left, e, side)
+            Rm = _mat(lib().oracle_euler_right, e, side)
+            want = -.5 * Rm @ ((fixed - np.abs(lam)) * (Lm @ (UR[:5] - UL[:5])))
+            assert np.allclose(_roe_fix(UL, UR, side) - _roe(UL, UR, side), want, rtol=1e-9, atol=1e-13)
+            n += 1
+    assert n == 6
+
+
 def _plm(Um, U, Up, limiter='minmod'):
     L = np.zeros(6)
     R = np.zeros(6)

@@ -12,6 +12,10 @@ namespace hydro {
 
 constexpr int NG = 2;              // numGhost (gridsolver.lua:41)
 constexpr int WAVE = 64;           // CDNA wavefront
+// Flux id of the kernels of the Roe flux with Harten's entropy fix
+// (roe.cl:83-106); a template choice of its own, so that the kernels without
+// the fix carry no test for it.
+constexpr int FLUX_ROE_ENTROPY_FIX = 16;
 constexpr int STRIP_OUT = WAVE;    // output columns per wave strip with edge passes
 // Strips whose edge interfaces come from an edge pass output 64 columns; the
 // others load a 2-cell halo in lanes 0-1 and 62-63 and output 60: 1D strips
@@ -103,8 +107,7 @@ struct StageArgs {
 	// equation parameters in the stencil type and in double (for the FP64
 	// accumulation, constrainU and dt work of the FP32 form)
 	R gamma, rhoMin, PMin, divPhi, divPsi;
-	R roeFixFactor;           // Roe, Euler: Harten's entropy fix when roeFix (roe.cl:83-106)
-	int roeFix;
+	R roeFixFactor;           // Roe, Euler: Harten's entropy fix factor (roe.cl:83-106)
 	double gammaD, rhoMinD, PMinD, divPhiD, divPsiD;
 	int limiter;              // flux or slope limiter index (runtime switch form)
 	int xcdRemap;

@@ -20,7 +20,7 @@
 namespace hydro {
 
 // FLUX: the interface flux the stage kernels build from this equation
-// (HYDRO_FLUX_ROE, _HLL, _RUSANOV, _HLLC or _MARQUINA).
+// (HYDRO_FLUX_ROE, _HLL, _RUSANOV, _HLLC, _MARQUINA, or FLUX_ROE_ENTROPY_FIX).
 template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE>
 struct EulerT {
 	template <typename R>
@@ -39,8 +39,7 @@ struct EulerT {
 
 		struct Par {
 			R gamma, g1, rhoMin, PMin;
-			R fixFactor;   // Roe: Harten's entropy fix when fix
-			bool fix;
+			R fixFactor;   // Roe: Harten's entropy fix factor (FLUX_ROE_ENTROPY_FIX kernels)
 		};
 		struct Prim {
 			R rho, v[3], P, inv;   // inv = 1 / rho
@@ -55,7 +54,6 @@ struct EulerT {
 			p.g1 = a.gamma - R(1);
 			p.rhoMin = a.rhoMin;
 			p.fixFactor = a.roeFixFactor;
-			p.fix = a.roeFix != 0;
 			p.PMin = a.PMin;
 			return p;
 		}

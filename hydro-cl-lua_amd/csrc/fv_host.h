@@ -407,7 +407,6 @@ static void stage_args(const Grid &g, int s, StageArgs<R> &a) {
 	a.PMinD = c.P_min;
 	a.divPhiD = c.div_phi_wavespeed;
 	a.divPsiD = c.div_psi_wavespeed;
-	a.roeFix = c.roe_entropy_fix != 0;
 	a.roeFixFactor = R(c.roe_entropy_fix_factor);
 	a.limiter = c.plm ? c.slope_limiter : c.flux_limiter;
 	a.resetSlot = -1;
@@ -423,6 +422,7 @@ static StageKey stage_key(const Grid &g) {
 	k.dim = g.dim;
 	k.plm = g.cfg.plm != HYDRO_PLM_NONE;
 	k.flux = g.cfg.flux;
+	k.roeFix = g.cfg.roe_entropy_fix != 0;
 	// only the Roe flux uses a flux limiter (fvsolver.lua:63)
 	k.lim = !k.plm && k.flux == HYDRO_FLUX_ROE && g.cfg.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
 	k.limiter = k.plm ? g.cfg.slope_limiter : g.cfg.flux_limiter;

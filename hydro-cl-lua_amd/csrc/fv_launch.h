@@ -20,6 +20,7 @@ struct StageKey {
 	int fast;       // FAST math policy
 	int nvel;       // Euler: active velocity components (dim or 3)
 	int flux;       // HYDRO_FLUX_*
+	int roeFix;     // Roe: Harten's entropy fix
 };
 
 // Limiters with a compile-time form; the rest use the runtime switch.

@@ -158,15 +158,13 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 #pragma unroll
 	for (int j = 0; j < E::NW; ++j) {
 		R lambda = I.lambda[j];
-		if constexpr (!E::SOURCE) {
+		if constexpr (E::FLUXID == FLUX_ROE_ENTROPY_FIX) {
 			// Harten's entropy fix (roe.cl:83-106): |lambda| below dws is
 			// replaced by (lambda^2/dws + dws)/2 with OpenCL's sign (0 at 0)
-			if (p.fix) {
-				const R dws = p.fixFactor * I.e.Cs;
-				if (fabs(lambda) < dws) {
-					const R sgn = lambda > R(0) ? R(1) : (lambda < R(0) ? R(-1) : lambda);
-					lambda = sgn * R(.5) * (E::M::div(lambda * lambda, dws) + dws);
-				}
+			const R dws = p.fixFactor * I.e.Cs;
+			if (fabs(lambda) < dws) {
+				const R sgn = lambda > R(0) ? R(1) : (lambda < R(0) ? R(-1) : lambda);
+				lambda = sgn * R(.5) * (E::M::div(lambda * lambda, dws) + dws);
 			}
 		}
 		R sgnLambda = lambda >= R(0) ? R(1) : R(-1);
@@ -205,7 +203,7 @@ template <typename R, class E, int S>
 __device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell<R, E> &UL, const Cell<R, E> &UR,
 	R dt_dx, int lim, R *F)
 {
-	if constexpr (E::FLUXID == HYDRO_FLUX_ROE) {
+	if constexpr (E::FLUXID == HYDRO_FLUX_ROE || E::FLUXID == FLUX_ROE_ENTROPY_FIX) {
 		Iface<R, E> I;
 		iface_prep<R, E, S>(p, UL, UR, I);
 		iface_flux<R, E, S, false, 0>(p, I, nullptr, nullptr, dt_dx, lim, F);

@@ -127,4 +127,111 @@ __global__ void __launch_bounds__(TOTALS_BLOCK) totals_partial_kernel(const R *U
 	}
 }
 
+// ------------------------------------------------------------ var stats
+// calcDisplayVarRangeAndAvgAndStdDev (solverbase.lua:2871-2994) of one
+// stored field: min and max over every cell of the ghosted grid (ghost values
+// as the boundary fills them: periodic/freeflow copies, mirror flips, fixed
+// values), the average and the population standard deviation over the
+// interior (the reference zeroes the ghosts for its sums and divides by the
+// interior count). Two passes, FP64 compensated sums.
+template <typename R, template <typename> class EQ>
+__global__ void __launch_bounds__(TOTALS_BLOCK) stats_partial_kernel(const GridDesc<R> g, int dim, int field, int pass,
+	const double *out, KSum *partial)
+{
+	using E = EQ<R>;
+	__shared__ KSum sh[TOTALS_BLOCK];
+	__shared__ KSum shm[TOTALS_BLOCK];
+	int gs[3];
+	for (int a = 0; a < 3; ++a) gs[a] = a < dim ? g.n[a] + 2 * NG : 1;
+	const long total = (long)gs[0] * gs[1] * gs[2];
+	const double avg = pass ? out[2] : 0.;
+	double mn = INFINITY, mx = -INFINITY;
+	KSum s = {0., 0.};
+	for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
+		int cc[3];
+		cc[0] = (int)(c % gs[0]) - NG;
+		cc[1] = dim > 1 ? (int)((c / gs[0]) % gs[1]) - NG : 0;
+		cc[2] = dim > 2 ? (int)(c / ((long)gs[0] * gs[1])) - NG : 0;
+		bool interior = true;
+		Mapped m[3];
+		for (int a = 0; a < 3; ++a) {
+			if (a < dim) {
+				interior = interior && cc[a] >= 0 && cc[a] < g.n[a];
+				m[a] = map_coord(cc[a], g.n[a], g.bc[2 * a], g.bc[2 * a + 1], 2 * a);
+			} else {
+				m[a].idx = 0; m[a].mirror = 0; m[a].fixedFace = -1;
+			}
+		}
+		if (pass && !interior) continue;
+		const long src = ((long)m[2].idx * g.n[1] + m[1].idx) * g.step1 + m[0].idx;
+		R v = field < E::Q ? g.U[0][field * g.ps + src] : g.stat[(field - E::Q) * g.ps + src];
+		// axis by axis as the boundary passes run: a fixed face writes its
+		// value, a mirror face flips the reflected fields
+		for (int a = 0; a < dim; ++a) {
+			if (m[a].fixedFace >= 0) v = g.fixedv[m[a].fixedFace][field];
+			else if (m[a].mirror && E::reflects(field, a)) v = -v;
+		}
+		const double x = (double)v;
+		if (pass == 0) {
+			mn = fmin(mn, x);
+			mx = fmax(mx, x);
+			if (interior) s = ksum_add(s, x);
+		} else {
+			const double d = x - avg;
+			s = ksum_add(s, d * d);
+		}
+	}
+	sh[threadIdx.x] = s;
+	shm[threadIdx.x] = KSum{mn, mx};
+	__syncthreads();
+	for (int w = TOTALS_BLOCK / 2; w > 0; w >>= 1) {
+		if ((int)threadIdx.x < w) {
+			sh[threadIdx.x] = ksum_merge(sh[threadIdx.x], sh[threadIdx.x + w]);
+			shm[threadIdx.x].s = fmin(shm[threadIdx.x].s, shm[threadIdx.x + w].s);
+			shm[threadIdx.x].c = fmax(shm[threadIdx.x].c, shm[threadIdx.x + w].c);
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		partial[blockIdx.x] = sh[0];
+		partial[gridDim.x + blockIdx.x] = shm[0];
+	}
+}
+
+// out = {min, max, avg, stddev}: pass 0 writes the first three, pass 1 the last
+[[maybe_unused]] static __global__ void __launch_bounds__(TOTALS_BLOCK) stats_final_kernel(const KSum *partial,
+	int nPartial, int pass, double nInterior, double *out)
+{
+	__shared__ KSum sh[TOTALS_BLOCK];
+	__shared__ KSum shm[TOTALS_BLOCK];
+	KSum a = {0., 0.};
+	KSum m = {INFINITY, -INFINITY};
+	for (int i = threadIdx.x; i < nPartial; i += blockDim.x) {
+		a = ksum_merge(a, partial[i]);
+		m.s = fmin(m.s, partial[nPartial + i].s);
+		m.c = fmax(m.c, partial[nPartial + i].c);
+	}
+	sh[threadIdx.x] = a;
+	shm[threadIdx.x] = m;
+	__syncthreads();
+	for (int w = TOTALS_BLOCK / 2; w > 0; w >>= 1) {
+		if ((int)threadIdx.x < w) {
+			sh[threadIdx.x] = ksum_merge(sh[threadIdx.x], sh[threadIdx.x + w]);
+			shm[threadIdx.x].s = fmin(shm[threadIdx.x].s, shm[threadIdx.x + w].s);
+			shm[threadIdx.x].c = fmax(shm[threadIdx.x].c, shm[threadIdx.x + w].c);
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		const double sum = sh[0].s + sh[0].c;
+		if (pass == 0) {
+			out[0] = shm[0].s;
+			out[1] = shm[0].c;
+			out[2] = sum / nInterior;
+		} else {
+			out[3] = sqrt(sum / nInterior);
+		}
+	}
+}
+
 } // namespace hydro

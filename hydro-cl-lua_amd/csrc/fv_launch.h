@@ -53,6 +53,12 @@ hipError_t launch_constrain(int eqn, int fast, const GridArgs<R> &g, hipStream_t
 template <typename R>
 hipError_t launch_totals(const R *U, long ps, long ncell, int q, KSum *partial, int nPartial, double *out, hipStream_t s);
 
+// min, max, average, standard deviation of one stored field (out: 4 doubles
+// on the device; partial: 2 nPartial KSums)
+template <typename R>
+hipError_t launch_stats(int eqn, int dim, const GridDesc<R> &g, int field, KSum *partial, int nPartial, double nInterior,
+	double *out, hipStream_t s);
+
 hipError_t launch_reset_slots(DevScalars *dev, int mask, hipStream_t s);
 hipError_t launch_fold_slot(DevScalars *dev, int slot, hipStream_t s);
 

@@ -235,6 +235,27 @@ extern "C" int hydro_set_time(hydro_solver *h, double t) {
 	return grid_set_time(h->g, t, h->stream);
 }
 
+extern "C" int hydro_var_stats(hydro_solver *h, int field, double *out) {
+	if (!h || !out) return fail("NULL argument");
+	Grid &g = h->g;
+	if (field < 0 || field >= g.Q + g.NSTAT) return fail("field %d out of range 0..%d", field, g.Q + g.NSTAT - 1);
+	double nInterior = (double)g.ncell;
+	hipError_t e;
+	if (g.elem == 8) {
+		GridDesc<double> d;
+		grid_desc<double>(g, d);
+		e = launch_stats<double>(g.cfg.eqn, g.dim, d, field, h->partial, h->nPartial, nInterior, h->totalsDev, h->stream);
+	} else {
+		GridDesc<float> d;
+		grid_desc<float>(g, d);
+		e = launch_stats<float>(g.cfg.eqn, g.dim, d, field, h->partial, h->nPartial, nInterior, h->totalsDev, h->stream);
+	}
+	if (e != hipSuccess) return fail("stats kernel: %s", hipGetErrorString(e));
+	HIPCHECK(hipMemcpyAsync(out, h->totalsDev, sizeof(double) * 4, hipMemcpyDeviceToHost, h->stream));
+	HIPCHECK(hipStreamSynchronize(h->stream));
+	return 0;
+}
+
 extern "C" int hydro_conserved_totals(hydro_solver *h, double *out) {
 	if (!h || !out) return fail("NULL argument");
 	Grid &g = h->g;

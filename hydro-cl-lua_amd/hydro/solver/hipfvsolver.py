@@ -164,6 +164,15 @@ class HipFVSolver:
         _abi.check(_abi.lib().hydro_conserved_totals(self.h, out))
         return np.array(out[:])
 
+    def calcDisplayVarRangeAndAvgAndStdDev(self, field):
+        """(min, max, avg, stddev) of the stored cons_t field `field` (index),
+        as SolverBase:calcDisplayVarRangeAndAvgAndStdDev (solverbase.lua:
+        2871-2994) gives them for the display var 'U <field>': min/max over
+        the ghosted grid, avg/stddev over the interior; computed on the GPU"""
+        out = (ctypes.c_double * 4)()
+        _abi.check(_abi.lib().hydro_var_stats(self.h, int(field), out))
+        return tuple(out[:])
+
     def timeUpdates(self, n, perKernel=True):
         t = _abi.HydroTiming()
         _abi.check(_abi.lib().hydro_time_updates(self.h, int(n), 1 if perKernel else 0, ctypes.byref(t)))

@@ -198,6 +198,16 @@ int hydro_set_time(hydro_solver *h, double t);
  * out has numIntStates entries. */
 int hydro_conserved_totals(hydro_solver *h, double *out);
 
+/* The display statistics of one stored cons_t field (0 .. numIntStates +
+ * static fields - 1), calcDisplayVarRangeAndAvgAndStdDev
+ * (hydro/solver/solverbase.lua:2871-2994) of the display var 'U <field>':
+ * out[0], out[1] = min and max over every cell of the ghosted grid, ghosts as
+ * the boundary fills them; out[2] = average and out[3] = population standard
+ * deviation over the interior (the reference zeroes the ghosts for its sums
+ * and divides by the interior count). Computed on the device in double
+ * (synchronizes). */
+int hydro_var_stats(hydro_solver *h, int field, double *out);
+
 /* Wait for the solver's stream. */
 int hydro_sync(hydro_solver *h);
 

@@ -970,6 +970,40 @@ static void boundary_buf(oracle_solver *s, real *buf) {
 
 void oracle_boundary(oracle_solver *s) { boundary_buf(s, s->U); }
 
+void oracle_var_stats(const oracle_solver *s, int field, double *out) {
+	/* solverbase.lua:2871-2994: reduceMin/reduceMax over the whole buffer;
+	 * reduceSum over the buffer with zeroed ghosts, divided by
+	 * volumeWithoutBorder; subtractAndSquare on the interior (solverbase.lua:
+	 * 1269-1278) for the standard deviation */
+	int S = s->S;
+	long n = 1, ni = 1;
+	for (int a = 0; a < 3; ++a) { n *= s->gs[a]; if (a < s->dim) ni *= s->gs[a] - 2 * NG; }
+	real mn = INFINITY, mx = -INFINITY, sum = 0;
+	for (long c = 0; c < n; ++c) {
+		real v = s->U[c * S + field];
+		mn = fmin(mn, v);
+		mx = fmax(mx, v);
+	}
+	int lo[3], hi[3];
+	for (int a = 0; a < 3; ++a) {
+		lo[a] = a < s->dim ? NG : 0;
+		hi[a] = a < s->dim ? s->gs[a] - NG : 1;
+	}
+	/* serial loops: one fixed summation order */
+	for (int k = lo[2]; k < hi[2]; ++k)
+		for (int j = lo[1]; j < hi[1]; ++j)
+			for (int i = lo[0]; i < hi[0]; ++i) sum += s->U[(i + s->step[1] * j + s->step[2] * k) * S + field];
+	real avg = sum / (real)ni;
+	real sq = 0;
+	for (int k = lo[2]; k < hi[2]; ++k)
+		for (int j = lo[1]; j < hi[1]; ++j)
+			for (int i = lo[0]; i < hi[0]; ++i) {
+				real d = s->U[(i + s->step[1] * j + s->step[2] * k) * S + field] - avg;
+				sq += d * d;
+			}
+	out[0] = mn; out[1] = mx; out[2] = avg; out[3] = sqrt(sq / (real)ni);
+}
+
 void oracle_constrain_U(oracle_solver *s) {
 	/* solverbase.lua:2116-2127: only equations with a constrainU module */
 	if (s->cfg.eqn != HYDRO_EQN_EULER) return;

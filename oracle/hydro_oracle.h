@@ -38,6 +38,10 @@ void oracle_set_threads(int n);
 int oracle_get_threads(void);
 
 void oracle_boundary(oracle_solver *s);
+/* calcDisplayVarRangeAndAvgAndStdDev (solverbase.lua:2871-2994) of cons_t
+ * field `field` of the ghosted state: out = {min, max over all cells, average,
+ * population standard deviation over the interior} */
+void oracle_var_stats(const oracle_solver *s, int field, double *out);
 void oracle_constrain_U(oracle_solver *s);      /* constrainU kernel + boundary */
 double oracle_calc_dt(oracle_solver *s);        /* cfl * min (or fixed dt) */
 void oracle_step(oracle_solver *s, double dt);  /* integrator step */

@@ -752,3 +752,23 @@ def test_glm_source_uniform_fields(plm):
     want[:3] = -sigma * np.array([.3, -.8, 1.1]) / eps
     want[6] = rhoq * 1.5
     assert np.allclose(d[..., :8], want, rtol=1e-14, atol=1e-14)
+
+
+def test_var_stats_are_min_max_mean_std():
+    """calcDisplayVarRangeAndAvgAndStdDev: min/max over the ghosted grid
+    (mirror ghosts flip the normal momenta, so they can hold the extremes),
+    numpy's mean and population std over the interior"""
+    cfg = make_cfg(dim=2, size=(20, 13), boundary='mirror', integrator='forward Euler', cfl=.3)
+    o = pyoracle.Oracle(cfg)
+    rng = np.random.default_rng(3)
+    o.set_U(init_euler.to_ghosted(cfg, init_euler.quadrant(cfg, init_euler.random_quadrant_states(rng))))
+    o.boundary()
+    o.constrain_U()
+    o.update(3)
+    out = np.zeros(4)
+    for f in range(5):
+        lib().oracle_var_stats(o.h, f, _ptr(out))
+        U, I = o.U[..., f], o.interior()[..., f]
+        assert out[0] == U.min() and out[1] == U.max()
+        assert np.isclose(out[2], I.mean(), rtol=1e-13, atol=1e-15)
+        assert np.isclose(out[3], I.std(), rtol=1e-12, atol=1e-15)

@@ -21,13 +21,15 @@ namespace hydro {
 
 // FLUX: the interface flux the stage kernels build from this equation
 // (HYDRO_FLUX_ROE, _HLL, _RUSANOV, _HLLC, _MARQUINA, or FLUX_ROE_ENTROPY_FIX).
-template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE>
+// PRIMPLM: 'plm prim' reconstruction (plm.cl:191-254) in place of 'plm cons'.
+template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE, bool PRIMPLM = false>
 struct EulerT {
 	template <typename R>
 	struct Eqn {
 		using M = Math<R, FAST>;
 		static constexpr bool FASTM = FAST;
 		static constexpr int FLUXID = FLUX;
+		static constexpr bool PLM_PRIM = PRIMPLM;
 		static constexpr int Q = 5;       // numIntStates (euler.lua:14)
 		static constexpr int NSTAT = 0;   // no static fields on the device
 		static constexpr int NW = 5;      // numWaves (euler.lua:13)
@@ -157,6 +159,14 @@ struct EulerT {
 			}
 			e.vSq = vSq;
 			return e;
+		}
+
+		// consFromPrim (euler.cl:161-173): m = v rho, ETotal = rho (v^2 / 2) + P / (gamma - 1)
+		__device__ static __forceinline__ void cons_from_prim(const Par &p, const R *W, R *U) {
+			U[0] = W[0];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) U[1 + i] = i < NVEL ? W[1 + i] * W[0] : R(0);
+			U[4] = W[0] * (R(.5) * lenSq3(W + 1)) + M::div(W[4], p.g1);
 		}
 
 		// eigen_forCell (euler.cl:315-341): one cell's own state, no guards

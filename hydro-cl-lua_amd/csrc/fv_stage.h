@@ -90,9 +90,29 @@ __device__ __forceinline__ void shift_right(const Cell<R, E> &in, Cell<R, E> &ou
 // plm.cl:27-91: one division per field (numerator / denominator chosen by the
 // sign of dUC, as the reference's two branches do)
 template <typename R, class E, int SLIM>
-__device__ __forceinline__ void plm_cons(int lim, const Cell<R, E> &Um, const Cell<R, E> &U,
+__device__ __forceinline__ void plm_cons(const typename E::Par &par, int lim, const Cell<R, E> &Um, const Cell<R, E> &U,
 	const Cell<R, E> &Up, Cell<R, E> &L, Cell<R, E> &Rr)
 {
+	if constexpr (E::PLM_PRIM) {
+		// 'plm prim' (plm.cl:191-254): the same slopes of the primitives
+		// (rho, v, P), r = dWL / dWR always, back to cons at the two edges
+		const auto Wm = E::prim(par, Um.v), W0 = E::prim(par, U.v), Wp = E::prim(par, Up.v);
+		const R wm[5] = {Wm.rho, Wm.v[0], Wm.v[1], Wm.v[2], Wm.P};
+		const R w0[5] = {W0.rho, W0.v[0], W0.v[1], W0.v[2], W0.P};
+		const R wp[5] = {Wp.rho, Wp.v[0], Wp.v[1], Wp.v[2], Wp.P};
+		R nL[5], nR[5];
+#pragma unroll
+		for (int j = 0; j < 5; ++j) {
+			const R dWR = wp[j] - w0[j];
+			const R dWL = w0[j] - wm[j];
+			const R sigma = limited<SLIM, E::FASTM>(lim, dWL, dWR);
+			nL[j] = w0[j] - R(.5) * sigma;
+			nR[j] = w0[j] + R(.5) * sigma;
+		}
+		E::cons_from_prim(par, nL, L.v);
+		E::cons_from_prim(par, nR, Rr.v);
+		return;
+	} else {
 #pragma unroll
 	for (int j = 0; j < Cell<R, E>::NV; ++j) {
 		if (j < E::Q) {
@@ -116,6 +136,7 @@ __device__ __forceinline__ void plm_cons(int lim, const Cell<R, E> &Um, const Ce
 			L.v[j] = U.v[j];
 			Rr.v[j] = U.v[j];
 		}
+	}
 	}
 }
 
@@ -348,8 +369,8 @@ __device__ __forceinline__ void edge_flux(const typename E::Par &par, int lim, i
 	R F[Q];
 	if constexpr (PLM) {
 		Cell<R, E> L1, R1, L2, R2;
-		plm_cons<R, E, SLIM>(slim, c[0], c[1], c[2], L1, R1);
-		plm_cons<R, E, SLIM>(slim, c[1], c[2], c[3], L2, R2);
+		plm_cons<R, E, SLIM>(par, slim, c[0], c[1], c[2], L1, R1);
+		plm_cons<R, E, SLIM>(par, slim, c[1], c[2], c[3], L2, R2);
 		iface_solve<R, E, 0>(par, R1, L2, dt_dx0, lim, F);
 #pragma unroll
 		for (int j = 0; j < E::NW; ++j) dW[j] = R(0);   // kept by the flux limiter only
@@ -440,7 +461,7 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 	R Fx[Q];
 	if constexpr (PLM) {
 		C Lx, Rx, RxL;
-		plm_cons<R, E, SLIM>(slim, Um, U, Up, Lx, Rx);
+		plm_cons<R, E, SLIM>(par, slim, Um, U, Up, Lx, Rx);
 		shift_left<R, E>(Rx, RxL);
 		if constexpr (EDGES) {
 			// as above: lane 0's interface is replaced, its left state only
@@ -592,7 +613,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				// W0 = U(r), W1 = U(r+1): cell r+1's edges, interface r+1/2
 				loadRow(r + 2, W2);
 				C L1, R1;
-				plm_cons<R, E, SLIM>(slim, W0, W1, W2, L1, R1);
+				plm_cons<R, E, SLIM>(par, slim, W0, W1, W2, L1, R1);
 				if (wantFlux) {
 					iface_solve<R, E, 1>(par, Ry0, L1, dt_dx1, lim, F);
 				} else {

@@ -29,7 +29,7 @@ INTEGRATOR_STAGES = [1, 2, 2, 2, 3, 4, 4, 2, 2, 3, 4, 4]
 BOUNDARIES = ['periodic', 'mirror', 'freeflow', 'fixed']
 EQNS = ['euler', 'glm_maxwell']
 FLUXES = ['roe', 'hll', 'rusanov', 'euler-hllc', 'marquina']
-PLMS = [None, 'plm cons']
+PLMS = [None, 'plm cons', 'plm prim']
 PRECISIONS = ['double', 'float']
 FACES = ['xmin', 'xmax', 'ymin', 'ymax', 'zmin', 'zmax']
 

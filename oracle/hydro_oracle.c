@@ -9,7 +9,7 @@
  *   HLL, Rusanov     hydro/flux/hll.cl:13-73 (hll.lua:10), rusanov.cl:13-39
  *   HLLC             hydro/flux/euler-hllc.cl:13-240 (hllcMethod 2)
  *   Marquina         hydro/flux/marquina.cl:5-93, eigen_forCell euler.cl:315-341
- *   plm cons         hydro/solver/plm.cl:27-91, 976-997
+ *   plm cons, prim   hydro/solver/plm.cl:27-91, 191-254, 976-997
  *   calcFlux         hydro/solver/fvsolver.lua:55-199
  *   calcDerivFromFlux hydro/solver/fvsolver.cl:1-125
  *   calcDT           hydro/eqn/cl/calcDT.cl:33-73, eqn.lua:1182-1227,
@@ -722,6 +722,29 @@ void oracle_rusanov_flux(const oracle_params *p, const double *UL, const double 
 	F[5] = 0;
 }
 
+void oracle_plm_prim(const oracle_params *p, int lim, const double *UL, const double *U, const double *UR,
+	double *L, double *R)
+{
+	/* plm.cl:191-254 ('plm prim', Euler): slopes of the primitive variables,
+	 * r = dWL / dWR always, back to cons at the edges */
+	real W[6], WL[6], WR[6], nWL[6], nWR[6];
+	oracle_euler_prim_from_cons(p, U, W);
+	oracle_euler_prim_from_cons(p, UL, WL);
+	oracle_euler_prim_from_cons(p, UR, WR);
+	for (int j = 0; j < 6; ++j) nWL[j] = nWR[j] = W[j];
+	for (int j = 0; j < 5; ++j) {
+		real dWR = WR[j] - W[j];
+		real dWL = W[j] - WL[j];
+		real r = dWR == 0 ? 0 : (dWL / dWR);
+		real phi = oracle_limiter(lim, r);
+		real sigma = phi * dWR;
+		nWL[j] -= .5 * sigma;
+		nWR[j] += .5 * sigma;
+	}
+	oracle_euler_cons_from_prim(p, nWL, L);
+	oracle_euler_cons_from_prim(p, nWR, R);
+}
+
 void oracle_plm_cons(int lim, int nint, int S, const double *UL, const double *U, const double *UR,
 	double *L, double *R)
 {
@@ -832,6 +855,9 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 	}
 	if (cfg->flux < HYDRO_FLUX_ROE || cfg->flux > HYDRO_FLUX_MARQUINA) {
 		set_err("bad flux"); return NULL;
+	}
+	if (cfg->plm == HYDRO_PLM_PRIM && cfg->eqn != HYDRO_EQN_EULER) {
+		set_err("plm prim reconstructs the Euler primitives"); return NULL;
 	}
 	if (cfg->flux != HYDRO_FLUX_ROE && cfg->eqn != HYDRO_EQN_EULER) {
 		set_err("HLL, Rusanov, HLLC and Marquina take the Euler equations"); return NULL;
@@ -1040,9 +1066,14 @@ static void calc_lr(oracle_solver *s, const real *U) {
 	FOR_CELLS(s, 1, 1, {
 		for (int side = 0; side < dim; ++side) {
 			real *lr = s->ULR + (side + dim * index) * 2 * S;
-			oracle_plm_cons(s->cfg.slope_limiter, s->q, S,
-				U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
-				lr, lr + S);
+			if (s->cfg.plm == HYDRO_PLM_PRIM)
+				oracle_plm_prim(&s->p, s->cfg.slope_limiter,
+					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
+					lr, lr + S);
+			else
+				oracle_plm_cons(s->cfg.slope_limiter, s->q, S,
+					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
+					lr, lr + S);
 		}
 	});
 }

@@ -84,6 +84,9 @@ void oracle_marquina_flux(const oracle_params *p, const double *UL, const double
 void oracle_roe_flux(const oracle_params *p, int eqn, int flux_limiter, double dt_dx,
 	const double *UL, const double *UR, const double *const *lim_args, int side, double *F);
 /* plm cons edge states of one cell along one axis (plm.cl:27-91) */
+/* 'plm prim' edge states (plm.cl:191-254), Euler; L and R get 6 reals */
+void oracle_plm_prim(const oracle_params *p, int lim, const double *UL, const double *U, const double *UR,
+	double *L, double *R);
 void oracle_plm_cons(int slope_limiter, int nint, int nstates,
 	const double *UL, const double *U, const double *UR, double *L, double *R);
 

@@ -416,6 +416,39 @@ def _plm(Um, U, Up, limiter='minmod'):
     return L, R
 
 
+def _plm_prim(Um, U, Up, limiter='minmod'):
+    L, Rr = np.zeros(6), np.zeros(6)
+    lib().oracle_plm_prim(ctypes.byref(P), LIMITERS.index(limiter), _ptr(Um), _ptr(U), _ptr(Up), _ptr(L), _ptr(Rr))
+    return L, Rr
+
+
+def _textbook_cons(rho, v, p, gamma=1.4):
+    v = np.asarray(v, float)
+    return np.array([rho, *(rho * v), .5 * rho * v @ v + p / (gamma - 1), 0.])
+
+
+def test_plm_prim_cases():
+    """'plm prim': constant data gives back U; primitives linear across the
+    three cells (minmod slope ratio 1) give the cons of W -+ dW/2 at the
+    edges; an extremum in every primitive gives back U"""
+    U = _cons(1.3, (.2, -.1, .05), .8)
+    L, Rr = _plm_prim(U, U, U)
+    assert np.allclose(L, U, rtol=1e-15) and np.allclose(Rr, U, rtol=1e-15)
+    W0 = np.array([1.3, .2, -.1, .05, .8])
+    dW = np.array([.1, -.03, .02, .01, .05])
+    Um = _textbook_cons(*(W0 - dW)[:1], (W0 - dW)[1:4], (W0 - dW)[4])
+    U0 = _textbook_cons(W0[0], W0[1:4], W0[4])
+    Up = _textbook_cons(*(W0 + dW)[:1], (W0 + dW)[1:4], (W0 + dW)[4])
+    L, Rr = _plm_prim(Um, U0, Up)
+    WL, WR = W0 - dW / 2, W0 + dW / 2
+    assert np.allclose(L, _textbook_cons(WL[0], WL[1:4], WL[4]), rtol=1e-14, atol=1e-15)
+    assert np.allclose(Rr, _textbook_cons(WR[0], WR[1:4], WR[4]), rtol=1e-14, atol=1e-15)
+    Wext = W0 + dW
+    Uext = _textbook_cons(Wext[0], Wext[1:4], Wext[4])
+    L, Rr = _plm_prim(U0, Uext, U0)
+    assert np.allclose(L, Uext, rtol=1e-14) and np.allclose(Rr, Uext, rtol=1e-14)
+
+
 def test_plm_cons_cases():
     c = np.array([1., 2, 3, 4, 5, 6])
     L, R = _plm(c, c, c)

@@ -12,6 +12,9 @@ Beside it on the same JSON line:
                 region, which holds only stage launches, over their count)
   fp32          the same KH problem in the FP32 bulk form, and its relative L1
                 difference per conserved variable from the FP64 result
+  fluxes        the same KH problem with the HLL, Rusanov, HLLC and Marquina
+                fluxes (plm cons / minmod)
+  reconstructions  the same KH problem, Roe flux, with 'plm prim' and 'plm eig'
   batched       512 independent 2D Euler four-quadrant grids (sides log-uniform
                 in [64, 512]), FP32, RK2-TVD, 300 steps: one persistent launch
                 per stage over all grids, hipGraph replay, autotuned launch
@@ -103,10 +106,10 @@ def cpu_baseline(seconds=10.0):
                       f'oracle = reference kernel sequence, AoS, unfused, OpenMP; cpu: {cpu}'}
 
 
-def kh_solver(n, real, fast, rows=0, flux='roe'):
+def kh_solver(n, real, fast, rows=0, flux='roe', plm='plm cons'):
     from hydro.solver.hipfvsolver import HipFVSolver
     from hydro.init import euler as init_euler
-    s = HipFVSolver(dim=2, gridSize=(n, n), mins=(-.5, -.5), maxs=(.5, .5), flux=flux, usePLM='plm cons',
+    s = HipFVSolver(dim=2, gridSize=(n, n), mins=(-.5, -.5), maxs=(.5, .5), flux=flux, usePLM=plm,
                     slopeLimiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4,
                     boundary='periodic', real=real, fastMath=fast)
     if rows:
@@ -292,7 +295,7 @@ def main():
     ap.add_argument('--size', type=int, default=N)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 KH leg')
-    ap.add_argument('--no-fluxes', action='store_true', help='skip the HLL / Rusanov / HLLC / Marquina KH leg')
+    ap.add_argument('--no-fluxes', action='store_true', help='skip the HLL / Rusanov / HLLC / Marquina and plm prim / plm eig KH legs')
     ap.add_argument('--no-batch', action='store_true', help='skip the batched leg')
     ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
     ap.add_argument('--no-3d', action='store_true', help='skip the 3D blast leg')
@@ -391,15 +394,18 @@ def main():
         del s32
     if not args.no_fluxes and args.real == 'double':
         # the same workload with the other flux plugins (plm cons/minmod, no flux limiter)
-        out['fluxes'] = {}
-        for flux in ('hll', 'rusanov', 'euler-hllc', 'marquina'):
-            sf = kh_solver(n, args.real, fast, args.rows, flux=flux)
+        # and the Roe flux with the other reconstructions ('plm prim', 'plm eig')
+        out['fluxes'], out['reconstructions'] = {}, {}
+        variants = [('fluxes', f, f, 'plm cons') for f in ('hll', 'rusanov', 'euler-hllc', 'marquina')]
+        variants += [('reconstructions', p, 'roe', p) for p in ('plm prim', 'plm eig')]
+        for leg, key, flux, plm in variants:
+            sf = kh_solver(n, args.real, fast, args.rows, flux=flux, plm=plm)
             steps_f = max(20, args.steps // 5)
             wallf, evf = timed_steps(sf, args.warmup, steps_f)
             avg_f = evf / (2 * steps_f)
-            out['fluxes'][flux] = {'value': cells * steps_f / wallf / 1e9, 'unit': 'Gcell-updates/s',
-                                   'steps': steps_f, 'avg_launch_ms': avg_f,
-                                   'roofline_frac': 5 * NQ * 8 / 2 * cells / (avg_f * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            out[leg][key] = {'value': cells * steps_f / wallf / 1e9, 'unit': 'Gcell-updates/s',
+                             'steps': steps_f, 'avg_launch_ms': avg_f,
+                             'roofline_frac': 5 * NQ * 8 / 2 * cells / (avg_f * 1e-3) / 1e9 / HBM_PEAK_GBS}
             del sf
     if not args.no_batch:
         out['batched'] = batched_leg(args.batch_grids, args.batch_steps)

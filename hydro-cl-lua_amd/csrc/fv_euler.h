@@ -21,15 +21,17 @@ namespace hydro {
 
 // FLUX: the interface flux the stage kernels build from this equation
 // (HYDRO_FLUX_ROE, _HLL, _RUSANOV, _HLLC, _MARQUINA, or FLUX_ROE_ENTROPY_FIX).
-// PRIMPLM: 'plm prim' reconstruction (plm.cl:191-254) in place of 'plm cons'.
-template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE, bool PRIMPLM = false>
+// PLMK: the reconstruction of the PLM kernels, HYDRO_PLM_CONS, HYDRO_PLM_PRIM
+// (plm.cl:191-254) or HYDRO_PLM_EIG (plm.cl:256-432).
+template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE, int PLMK = HYDRO_PLM_CONS>
 struct EulerT {
 	template <typename R>
 	struct Eqn {
 		using M = Math<R, FAST>;
 		static constexpr bool FASTM = FAST;
 		static constexpr int FLUXID = FLUX;
-		static constexpr bool PLM_PRIM = PRIMPLM;
+		static constexpr bool PLM_PRIM = PLMK == HYDRO_PLM_PRIM;
+		static constexpr bool PLM_EIG = PLMK == HYDRO_PLM_EIG;
 		static constexpr int Q = 5;       // numIntStates (euler.lua:14)
 		static constexpr int NSTAT = 0;   // no static fields on the device
 		static constexpr int NW = 5;      // numWaves (euler.lua:13)

@@ -168,10 +168,12 @@ static int validate(const hydro_cfg *c) {
 		return fail("roe_entropy_fix_factor must be > 0");
 	if (c->flux_limiter < 0 || c->flux_limiter >= HYDRO_NUM_LIMITERS) return fail("flux_limiter %d out of range", c->flux_limiter);
 	if (c->slope_limiter < 0 || c->slope_limiter >= HYDRO_NUM_LIMITERS) return fail("slope_limiter %d out of range", c->slope_limiter);
-	if (c->plm != HYDRO_PLM_NONE && c->plm != HYDRO_PLM_CONS && c->plm != HYDRO_PLM_PRIM)
+	if (c->plm != HYDRO_PLM_NONE && c->plm != HYDRO_PLM_CONS && c->plm != HYDRO_PLM_PRIM && c->plm != HYDRO_PLM_EIG)
 		return fail("plm %d not supported", c->plm);
-	if (c->plm == HYDRO_PLM_PRIM && (c->eqn != HYDRO_EQN_EULER || c->flux != HYDRO_FLUX_ROE || c->roe_entropy_fix))
-		return fail("plm prim is built for the Euler equations with the Roe flux (no entropy fix)");
+	if ((c->plm == HYDRO_PLM_PRIM || c->plm == HYDRO_PLM_EIG)
+		&& (c->eqn != HYDRO_EQN_EULER || c->flux != HYDRO_FLUX_ROE || c->roe_entropy_fix))
+		return fail("%s is built for the Euler equations with the Roe flux (no entropy fix)",
+			c->plm == HYDRO_PLM_PRIM ? "plm prim" : "plm eig");
 	// gridsolver.lua:119
 	if (c->plm && c->flux_limiter != HYDRO_LIMITER_DONOR_CELL)
 		return fail("are you sure you want to use flux and slope limiters at the same time?");
@@ -426,7 +428,7 @@ static StageKey stage_key(const Grid &g) {
 	k.plm = g.cfg.plm != HYDRO_PLM_NONE;
 	k.flux = g.cfg.flux;
 	k.roeFix = g.cfg.roe_entropy_fix != 0;
-	k.plmPrim = g.cfg.plm == HYDRO_PLM_PRIM;
+	k.plmKind = g.cfg.plm;
 	// only the Roe flux uses a flux limiter (fvsolver.lua:63)
 	k.lim = !k.plm && k.flux == HYDRO_FLUX_ROE && g.cfg.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
 	k.limiter = k.plm ? g.cfg.slope_limiter : g.cfg.flux_limiter;

@@ -88,12 +88,62 @@ __device__ __forceinline__ void shift_right(const Cell<R, E> &in, Cell<R, E> &ou
 
 // ------------------------------------------------------------- plm cons
 // plm.cl:27-91: one division per field (numerator / denominator chosen by the
-// sign of dUC, as the reference's two branches do)
-template <typename R, class E, int SLIM>
+// sign of dUC, as the reference's two branches do). S: the axis of the
+// reconstruction, dt_dx = dt / dx along it (used by 'plm eig' only).
+template <typename R, class E, int SLIM, int S>
 __device__ __forceinline__ void plm_cons(const typename E::Par &par, int lim, const Cell<R, E> &Um, const Cell<R, E> &U,
-	const Cell<R, E> &Up, Cell<R, E> &L, Cell<R, E> &Rr)
+	const Cell<R, E> &Up, Cell<R, E> &L, Cell<R, E> &Rr, R dt_dx)
 {
-	if constexpr (E::PLM_PRIM) {
+	if constexpr (E::PLM_EIG) {
+		// 'plm eig' (plm.cl:256-432, its enabled form): the slopes in the
+		// characteristic variables of the cell's own eigensystem, limited
+		// with r = dL / dR, each wave's slope kept on the side it moves
+		// towards, back to cons, then half a step of the edge fluxes
+		constexpr int Q = E::Q;
+		R dUL[Q], dUR[Q];
+#pragma unroll
+		for (int j = 0; j < Q; ++j) {
+			dUL[j] = U.v[j] - Um.v[j];
+			dUR[j] = Up.v[j] - U.v[j];
+		}
+		const auto W = E::prim(par, U.v);
+		const auto e = E::eigen_cell(par, U.v, W);
+		R aL[E::NW], aR[E::NW], lam[E::NW];
+		E::template left<S>(par, e, dUL, aL);
+		E::template left<S>(par, e, dUR, aR);
+		E::template waves<S>(par, e, lam);
+#pragma unroll
+		for (int j = 0; j < E::NW; ++j) {
+			const R sigma = limited<SLIM, E::FASTM>(lim, aL[j], aR[j]);
+			aR[j] = lam[j] >= R(0) ? R(0) : sigma;
+			aL[j] = lam[j] <= R(0) ? R(0) : sigma;
+		}
+		R sL[Q], sR[Q];
+		E::template right<S>(par, e, aL, sL);
+		E::template right<S>(par, e, aR, sR);
+		R hL[Q], hR[Q];
+#pragma unroll
+		for (int j = 0; j < Q; ++j) {
+			hL[j] = U.v[j] - R(.5) * sL[j];
+			hR[j] = U.v[j] + R(.5) * sR[j];
+		}
+		R FL[Q], FR[Q];
+		E::template flux<S>(par, hL, E::prim(par, hL), FL);
+		E::template flux<S>(par, hR, E::prim(par, hR), FR);
+		const R h = R(.5) * dt_dx;
+#pragma unroll
+		for (int j = 0; j < Cell<R, E>::NV; ++j) {
+			if (j < Q) {
+				const R dF = FR[j] - FL[j];
+				L.v[j] = hL[j] + h * dF;
+				Rr.v[j] = hR[j] + h * dF;
+			} else {
+				L.v[j] = U.v[j];
+				Rr.v[j] = U.v[j];
+			}
+		}
+		return;
+	} else if constexpr (E::PLM_PRIM) {
 		// 'plm prim' (plm.cl:191-254): the same slopes of the primitives
 		// (rho, v, P), r = dWL / dWR always, back to cons at the two edges
 		const auto Wm = E::prim(par, Um.v), W0 = E::prim(par, U.v), Wp = E::prim(par, Up.v);
@@ -369,8 +419,8 @@ __device__ __forceinline__ void edge_flux(const typename E::Par &par, int lim, i
 	R F[Q];
 	if constexpr (PLM) {
 		Cell<R, E> L1, R1, L2, R2;
-		plm_cons<R, E, SLIM>(par, slim, c[0], c[1], c[2], L1, R1);
-		plm_cons<R, E, SLIM>(par, slim, c[1], c[2], c[3], L2, R2);
+		plm_cons<R, E, SLIM, 0>(par, slim, c[0], c[1], c[2], L1, R1, dt_dx0);
+		plm_cons<R, E, SLIM, 0>(par, slim, c[1], c[2], c[3], L2, R2, dt_dx0);
 		iface_solve<R, E, 0>(par, R1, L2, dt_dx0, lim, F);
 #pragma unroll
 		for (int j = 0; j < E::NW; ++j) dW[j] = R(0);   // kept by the flux limiter only
@@ -461,7 +511,7 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 	R Fx[Q];
 	if constexpr (PLM) {
 		C Lx, Rx, RxL;
-		plm_cons<R, E, SLIM>(par, slim, Um, U, Up, Lx, Rx);
+		plm_cons<R, E, SLIM, 0>(par, slim, Um, U, Up, Lx, Rx, dt_dx0);
 		shift_left<R, E>(Rx, RxL);
 		if constexpr (EDGES) {
 			// as above: lane 0's interface is replaced, its left state only
@@ -613,7 +663,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				// W0 = U(r), W1 = U(r+1): cell r+1's edges, interface r+1/2
 				loadRow(r + 2, W2);
 				C L1, R1;
-				plm_cons<R, E, SLIM>(par, slim, W0, W1, W2, L1, R1);
+				plm_cons<R, E, SLIM, 1>(par, slim, W0, W1, W2, L1, R1, dt_dx1);
 				if (wantFlux) {
 					iface_solve<R, E, 1>(par, Ry0, L1, dt_dx1, lim, F);
 				} else {

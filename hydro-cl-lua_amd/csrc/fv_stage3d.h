@@ -191,7 +191,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			if constexpr (PLM) {
 				loadPlane(r + 2, W2);
 				C L1, R1;
-				plm_cons<R, E, SLIM>(par, slim, W0, W1, W2, L1, R1);
+				plm_cons<R, E, SLIM, 2>(par, slim, W0, W1, W2, L1, R1, dt_dx2);
 				if (wantFlux) {
 					iface_solve<R, E, 2>(par, Rz0, L1, dt_dx2, lim, F);
 				} else {
@@ -244,7 +244,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				C Um, Up, Ly, Ry;
 				lds_get<R, E, NV>(sh.u[w + 1], Um.v, lane);
 				lds_get<R, E, NV>(sh.u[w + 3], Up.v, lane);
-				plm_cons<R, E, SLIM>(par, slim, Um, U, Up, Ly, Ry);
+				plm_cons<R, E, SLIM, 1>(par, slim, Um, U, Up, Ly, Ry, dt_dx1);
 				lds_put<R, E, NV>(sh.e[w + 1], Ry.v, lane);
 				C Ltop;
 				if (w == 0) {
@@ -252,7 +252,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 					lds_get<R, E, NV>(sh.u[0], a0.v, lane);
 					lds_get<R, E, NV>(sh.u[1], a1.v, lane);
 					lds_get<R, E, NV>(sh.u[2], a2.v, lane);
-					plm_cons<R, E, SLIM>(par, slim, a0, a1, a2, Le, Re);
+					plm_cons<R, E, SLIM, 1>(par, slim, a0, a1, a2, Le, Re, dt_dx1);
 					lds_put<R, E, NV>(sh.e[0], Re.v, lane);
 				}
 				if (w == W - 1) {
@@ -260,7 +260,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 					lds_get<R, E, NV>(sh.u[W + 1], a0.v, lane);
 					lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
 					lds_get<R, E, NV>(sh.u[W + 3], a2.v, lane);
-					plm_cons<R, E, SLIM>(par, slim, a0, a1, a2, Ltop, Re);
+					plm_cons<R, E, SLIM, 1>(par, slim, a0, a1, a2, Ltop, Re, dt_dx1);
 				}
 				__syncthreads();
 				C Rm;

@@ -29,7 +29,11 @@ INTEGRATOR_STAGES = [1, 2, 2, 2, 3, 4, 4, 2, 2, 3, 4, 4]
 BOUNDARIES = ['periodic', 'mirror', 'freeflow', 'fixed']
 EQNS = ['euler', 'glm_maxwell']
 FLUXES = ['roe', 'hll', 'rusanov', 'euler-hllc', 'marquina']
-PLMS = [None, 'plm cons', 'plm prim']
+PLMS = [None, 'plm cons', 'plm prim', 'plm eig']
+# plm.cl:10-24: the edge states are the cell's own, the flux of no
+# reconstruction (which the flux limiter check of gridsolver.lua:119 keeps at
+# 'donor cell'), so it runs as plm None
+PLM_PIECEWISE_CONSTANT = 'piecewise constant'
 PRECISIONS = ['double', 'float']
 FACES = ['xmin', 'xmax', 'ymin', 'ymax', 'zmin', 'zmax']
 
@@ -102,6 +106,10 @@ def make_cfg(dim, size, mins=None, maxs=None, eqn='euler', flux='roe',
     c.eqn = _index(EQNS, eqn, 'eqn')
     c.flux = _index(FLUXES, flux, 'flux')
     c.flux_limiter = _index(LIMITERS, flux_limiter, 'flux limiter')
+    if plm == PLM_PIECEWISE_CONSTANT:
+        if _index(LIMITERS, flux_limiter, 'flux limiter') != 0:
+            raise ValueError('are you sure you want to use flux and slope limiters at the same time?')
+        plm = None
     c.plm = _index(PLMS, plm, 'plm') if not isinstance(plm, bool) else int(plm)
     c.slope_limiter = _index(LIMITERS, slope_limiter, 'slope limiter')
     c.integrator = _index(INTEGRATORS, integrator, 'integrator')

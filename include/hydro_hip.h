@@ -65,7 +65,12 @@ enum {
 };
 
 /* Reconstruction (solver.usePLM): none, or 'plm cons' (hydro/solver/plm.cl:27-91). */
-enum { HYDRO_PLM_NONE = 0, HYDRO_PLM_CONS = 1, HYDRO_PLM_PRIM = 2 /* Euler, Roe flux (plm.cl:191-254) */ };
+enum {
+	HYDRO_PLM_NONE = 0,
+	HYDRO_PLM_CONS = 1,
+	HYDRO_PLM_PRIM = 2,   /* Euler, Roe flux (plm.cl:191-254) */
+	HYDRO_PLM_EIG = 3     /* Euler, Roe flux (plm.cl:256-432) */
+};
 
 /* Explicit integrators, hydro/int/fe.lua and the Shu-Osher tables of
  * hydro/int/all.lua:10-160 (same names). Implicit ones are not provided. */

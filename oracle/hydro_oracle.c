@@ -745,6 +745,62 @@ void oracle_plm_prim(const oracle_params *p, int lim, const double *UL, const do
 	oracle_euler_cons_from_prim(p, nWR, R);
 }
 
+void oracle_plm_eig(const oracle_params *p, int lim, const double *UL, const double *U, const double *UR,
+	int s, double dt_dx, double *L, double *R)
+{
+	/* plm.cl:256-432 ('plm eig', the form its '#if 1' enables), Euler,
+	 * Cartesian: the slopes dUL = U - UL, dUR = UR - U in the characteristic
+	 * variables of the cell's own eigensystem (eigen_forCell), limited with
+	 * r = dL / dR (0 when dR == 0) to sigma = phi(r) dR; a wave with speed >= 0
+	 * keeps sigma on the left edge only, a wave with speed <= 0 on the right
+	 * edge only; back to cons, UHalfL = U - sL / 2, UHalfR = U + sR / 2; both
+	 * edges then add (dt/dx)/2 (F(UHalfR) - F(UHalfL)). The center slope's
+	 * transform (dUCEig) is computed there but used by no output. */
+	real dUL[6], dUR[6];
+	for (int j = 0; j < 5; ++j) {
+		dUL[j] = U[j] - UL[j];
+		dUR[j] = UR[j] - U[j];
+	}
+	dUL[5] = dUR[5] = 0;
+	real eig[10];
+	oracle_euler_eigen_for_cell(p, U, eig);
+	real dULEig[5], dUREig[5];
+	oracle_euler_left(p, eig, dUL, s, dULEig);
+	oracle_euler_left(p, eig, dUR, s, dUREig);
+	real wave[5];
+	oracle_euler_waves(p, eig, s, wave);
+	for (int j = 0; j < 5; ++j) {
+		real rEig = dUREig[j] == 0 ? 0 : (dULEig[j] / dUREig[j]);
+		real phi = oracle_limiter(lim, rEig);
+		real sigma = phi * dUREig[j];
+		dULEig[j] = sigma;
+		dUREig[j] = sigma;
+		if (wave[j] >= 0) dUREig[j] = 0;
+		if (wave[j] <= 0) dULEig[j] = 0;
+	}
+	real sL[6], sR[6];
+	oracle_euler_right(p, eig, dULEig, s, sL);
+	oracle_euler_right(p, eig, dUREig, s, sR);
+	real UHalfL[6], UHalfR[6];
+	for (int j = 0; j < 6; ++j) UHalfL[j] = UHalfR[j] = U[j];
+	for (int j = 0; j < 5; ++j) {
+		UHalfL[j] -= .5 * sL[j];
+		UHalfR[j] += .5 * sR[j];
+	}
+	real FHalfL[6], FHalfR[6];
+	oracle_euler_flux_from_cons(p, UHalfL, s, FHalfL);
+	oracle_euler_flux_from_cons(p, UHalfR, s, FHalfR);
+	for (int j = 0; j < 6; ++j) {
+		L[j] = UHalfL[j];
+		R[j] = UHalfR[j];
+	}
+	for (int j = 0; j < 5; ++j) {
+		real dF = FHalfR[j] - FHalfL[j];
+		L[j] += .5 * dt_dx * dF;
+		R[j] += .5 * dt_dx * dF;
+	}
+}
+
 void oracle_plm_cons(int lim, int nint, int S, const double *UL, const double *U, const double *UR,
 	double *L, double *R)
 {
@@ -856,8 +912,8 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 	if (cfg->flux < HYDRO_FLUX_ROE || cfg->flux > HYDRO_FLUX_MARQUINA) {
 		set_err("bad flux"); return NULL;
 	}
-	if (cfg->plm == HYDRO_PLM_PRIM && cfg->eqn != HYDRO_EQN_EULER) {
-		set_err("plm prim reconstructs the Euler primitives"); return NULL;
+	if ((cfg->plm == HYDRO_PLM_PRIM || cfg->plm == HYDRO_PLM_EIG) && cfg->eqn != HYDRO_EQN_EULER) {
+		set_err("plm prim and plm eig are built for the Euler equations"); return NULL;
 	}
 	if (cfg->flux != HYDRO_FLUX_ROE && cfg->eqn != HYDRO_EQN_EULER) {
 		set_err("HLL, Rusanov, HLLC and Marquina take the Euler equations"); return NULL;
@@ -1061,12 +1117,17 @@ double oracle_calc_dt(oracle_solver *s) {
 	return s->cfg.cfl * m;
 }
 
-static void calc_lr(oracle_solver *s, const real *U) {
+static void calc_lr(oracle_solver *s, const real *U, real dt) {
+	/* calcLR (fvsolver.lua:232-234) with the stage's dt */
 	int S = s->S, dim = s->dim;
 	FOR_CELLS(s, 1, 1, {
 		for (int side = 0; side < dim; ++side) {
 			real *lr = s->ULR + (side + dim * index) * 2 * S;
-			if (s->cfg.plm == HYDRO_PLM_PRIM)
+			if (s->cfg.plm == HYDRO_PLM_EIG)
+				oracle_plm_eig(&s->p, s->cfg.slope_limiter,
+					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
+					side, dt / s->dx[side], lr, lr + S);
+			else if (s->cfg.plm == HYDRO_PLM_PRIM)
 				oracle_plm_prim(&s->p, s->cfg.slope_limiter,
 					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
 					lr, lr + S);
@@ -1175,7 +1236,7 @@ void oracle_calc_deriv(oracle_solver *s, double dt, double *deriv) {
 	 * (solverbase.lua:3208-3214); the derivative buffer is cleared first
 	 * (int.lua:6-8) */
 	memset(deriv, 0, s->G * s->S * sizeof(real));
-	if (s->cfg.plm) calc_lr(s, s->U);
+	if (s->cfg.plm) calc_lr(s, s->U, dt);
 	calc_flux(s, s->U, dt);
 	calc_deriv_from_flux(s, deriv);
 	if (s->cfg.eqn == HYDRO_EQN_GLM_MAXWELL) glm_add_source(s, deriv, s->U);

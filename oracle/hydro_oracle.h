@@ -87,6 +87,10 @@ void oracle_roe_flux(const oracle_params *p, int eqn, int flux_limiter, double d
 /* 'plm prim' edge states (plm.cl:191-254), Euler; L and R get 6 reals */
 void oracle_plm_prim(const oracle_params *p, int lim, const double *UL, const double *U, const double *UR,
 	double *L, double *R);
+/* 'plm eig' edge states (plm.cl:256-432), Euler, along axis side with
+ * dt_dx = dt / dx; L and R get 6 reals */
+void oracle_plm_eig(const oracle_params *p, int lim, const double *UL, const double *U, const double *UR,
+	int side, double dt_dx, double *L, double *R);
 void oracle_plm_cons(int slope_limiter, int nint, int nstates,
 	const double *UL, const double *U, const double *UR, double *L, double *R);
 

@@ -47,3 +47,26 @@ def test_create_rejects_plm_with_flux_limiter():
     rc = L.hydro_create(ctypes.byref(c), ctypes.byref(h))
     assert rc != 0
     assert b'flux and slope limiters' in L.hydro_last_error()
+
+
+def test_create_rejects_plm_kinds_outside_euler_roe():
+    """'plm prim' and 'plm eig' are built for the Euler equations with the Roe
+    flux; other pairings are refused before any device work"""
+    L = _abi.lib()
+    for kw in (dict(eqn='glm_maxwell', plm='plm eig'), dict(flux='hll', plm='plm eig'),
+               dict(plm='plm eig', use_entropy_flux_fix=True), dict(eqn='glm_maxwell', plm='plm prim')):
+        c = make_cfg(dim=2, size=(8, 8), **kw)
+        h = ctypes.c_void_p()
+        assert L.hydro_create(ctypes.byref(c), ctypes.byref(h)) != 0, kw
+        assert b'built for the Euler equations' in L.hydro_last_error() or b'entropy fix' in L.hydro_last_error()
+
+
+def test_piecewise_constant_is_no_reconstruction():
+    """plm.cl:10-24: 'piecewise constant' edge states are the cell's own, so
+    the configuration is plm None; a flux limiter with it is refused as
+    gridsolver.lua:119 does"""
+    import pytest
+    c = make_cfg(dim=2, size=(8, 8), plm='piecewise constant')
+    assert c.plm == 0 and c.flux_limiter == 0
+    with pytest.raises(ValueError, match='flux and slope limiters'):
+        make_cfg(dim=2, size=(8, 8), plm='piecewise constant', flux_limiter='superbee')

@@ -161,30 +161,37 @@ def test_roe_entropy_fix(rec, fast):
     assert not np.array_equal(outs[0], outs[1])
 
 
+PLM_KINDS = ['plm prim', 'plm eig']
+
+
 @pytest.mark.parametrize('real,fast', [('double', False), ('double', True), ('float', True)])
-def test_plm_prim_2d(real, fast):
-    """'plm prim' (plm.cl:191-254) with the Roe flux: slopes of the primitives"""
-    kw = dict(dim=2, gridSize=(150, 45), usePLM='plm prim', slopeLimiter='van Leer', integrator='Runge-Kutta 2, TVD',
+@pytest.mark.parametrize('plm', PLM_KINDS)
+def test_plm_kinds_2d(plm, real, fast):
+    """'plm prim' (plm.cl:191-254): slopes of the primitives; 'plm eig'
+    (plm.cl:256-432): limited characteristic slopes with half a step of the
+    edge fluxes; both with the Roe flux"""
+    kw = dict(dim=2, gridSize=(150, 45), usePLM=plm, slopeLimiter='van Leer', integrator='Runge-Kutta 2, TVD',
               cfl=.3, boundary={'xmin': 'mirror', 'xmax': 'freeflow', 'ymin': 'periodic', 'ymax': 'periodic'})
     s, o = _pair(_quad(12), 8, real=real, fastMath=fast, **kw)
-    _check(s, o, 2, TOL64 if real == 'double' else TOL32, 8, f'plm-prim-{real}-{fast}')
+    _check(s, o, 2, TOL64 if real == 'double' else TOL32, 8, f'{plm}-{real}-{fast}')
 
 
-def test_plm_prim_1d_3d_batch():
-    kw = dict(dim=1, gridSize=256, usePLM='plm prim', slopeLimiter='minmod', integrator='Runge-Kutta 3, TVD',
+@pytest.mark.parametrize('plm', PLM_KINDS)
+def test_plm_kinds_1d_3d_batch(plm):
+    kw = dict(dim=1, gridSize=256, usePLM=plm, slopeLimiter='minmod', integrator='Runge-Kutta 3, TVD',
               cfl=.5, boundary='freeflow')
     s, o = _pair(_sod, 30, **kw)
-    _check(s, o, 1, TOL64, 30, 'plm-prim-1d')
+    _check(s, o, 1, TOL64, 30, f'{plm}-1d')
     import pyoracle
     from hydro.solver.hipfvsolver import HipFVSolver
     from hydro.init import euler as ie
     from test_gpu_3d import _octants
     box = dict(mins=(-1, -1, -1), maxs=(1, 1, 1))
-    s = HipFVSolver(dim=3, gridSize=(70, 13, 9), usePLM='plm prim', slopeLimiter='minmod',
+    s = HipFVSolver(dim=3, gridSize=(70, 13, 9), usePLM=plm, slopeLimiter='minmod',
                     integrator='Runge-Kutta 2, TVD', cfl=.3, boundary='mirror', **box)
     U0 = ie.to_ghosted(s.cfg, _octants(6)(s.cfg))
     s.resetState(U0)
-    o = pyoracle.Oracle(pyoracle.make_cfg(dim=3, size=(70, 13, 9), plm='plm prim', slope_limiter='minmod',
+    o = pyoracle.Oracle(pyoracle.make_cfg(dim=3, size=(70, 13, 9), plm=plm, slope_limiter='minmod',
                                           integrator='Runge-Kutta 2, TVD', cfl=.3, boundary='mirror',
                                           precision='double', **box))
     o.set_U(U0)
@@ -193,11 +200,11 @@ def test_plm_prim_1d_3d_batch():
     s.update(4)
     o.update(4)
     err = rel_l1(o.interior(s.UBufObj.toCPU())[..., :5], o.interior()[..., :5])
-    print('plm-prim-3d', err)
+    print(plm, '3d', err)
     assert np.all(err <= TOL64), err
     from hydro.solver.batch import HipBatch
     from hydro._cfg import make_cfg
-    kws = [dict(dim=2, size=n, mins=(-1, -1), maxs=(1, 1), plm='plm prim', slope_limiter='minmod',
+    kws = [dict(dim=2, size=n, mins=(-1, -1), maxs=(1, 1), plm=plm, slope_limiter='minmod',
                 integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='freeflow', precision='double', fast_math=True)
            for n in [(64, 64), (100, 37)]]
     b = HipBatch(kws)
@@ -209,7 +216,7 @@ def test_plm_prim_1d_3d_batch():
     b.constrainU()
     b.update(5)
     for i, kwi in enumerate(kws):
-        sp = HipFVSolver(dim=2, gridSize=kwi['size'], mins=kwi['mins'], maxs=kwi['maxs'], usePLM='plm prim',
+        sp = HipFVSolver(dim=2, gridSize=kwi['size'], mins=kwi['mins'], maxs=kwi['maxs'], usePLM=plm,
                          slopeLimiter='minmod', integrator=kwi['integrator'], cfl=kwi['cfl'], boundary='freeflow',
                          fastMath=True)
         sp.resetState(U0s[i])

@@ -449,6 +449,119 @@ def test_plm_prim_cases():
     assert np.allclose(L, Uext, rtol=1e-14) and np.allclose(Rr, Uext, rtol=1e-14)
 
 
+def _plm_eig(Um, U, Up, side=0, dt_dx=.3, limiter='minmod'):
+    L, Rr = np.zeros(6), np.zeros(6)
+    lib().oracle_plm_eig(ctypes.byref(P), LIMITERS.index(limiter), _ptr(np.asarray(Um, float)),
+                         _ptr(np.asarray(U, float)), _ptr(np.asarray(Up, float)), side, dt_dx, _ptr(L), _ptr(Rr))
+    return L, Rr
+
+
+def _flux_x(U, gamma=1.4):
+    """textbook Euler flux along x of (rho, mx, my, mz, E)"""
+    rho, m, E = U[0], np.asarray(U[1:4]), U[4]
+    v = m / rho
+    p = (gamma - 1) * (E - .5 * rho * v @ v)
+    return np.array([m[0], m[0] * v[0] + p, m[1] * v[0], m[2] * v[0], (E + p) * v[0]])
+
+
+def _jacobian_1d(U, gamma=1.4):
+    """textbook Jacobian of the 1D Euler flux in (rho, m, E)"""
+    rho, m, E = U
+    u = m / rho
+    H = (E + (gamma - 1) * (E - .5 * m * u)) / rho
+    return np.array([[0, 1, 0],
+                     [(gamma - 3) / 2 * u * u, (3 - gamma) * u, gamma - 1],
+                     [u * ((gamma - 1) / 2 * u * u - H), H - (gamma - 1) * u * u, gamma * u]])
+
+
+_PHI = {'minmod': lambda r: max(0., min(1., r)),
+        'van Leer': lambda r: (r + abs(r)) / (1 + abs(r)),
+        'superbee': lambda r: max(0., min(2 * r, 1.), min(r, 2.))}
+
+
+def test_plm_eig_constant_and_supersonic():
+    """'plm eig': constant data gives back U. Cons data linear across the
+    three cells (slope d, minmod ratio 1) in a state whose five waves all move
+    right keeps the whole slope on the left edge: L = U - d/2 + h, R = U + h
+    with h = (dt/dx)/2 (F(U) - F(U - d/2)); all waves moving left, the mirror
+    image: L = U + h, R = U + d/2 + h with h = (dt/dx)/2 (F(U + d/2) - F(U))."""
+    U = _cons(1.3, (.2, -.1, .05), .8)
+    L, Rr = _plm_eig(U, U, U)
+    assert np.array_equal(L, U) and np.array_equal(Rr, U)
+    d = np.array([.02, -.01, .015, .005, .03, 0.])
+    dt_dx = .37
+    for vx in (3., -3.):   # |vx| > Cs = sqrt(1.4 * .8 / 1.3) ~ .93
+        U = _cons(1.3, (vx, -.1, .05), .8)
+        L, Rr = _plm_eig(U - d, U, U + d, 0, dt_dx)
+        if vx > 0:
+            h = .5 * dt_dx * (_flux_x(U[:5]) - _flux_x((U - d / 2)[:5]))
+            wantL, wantR = U[:5] - d[:5] / 2 + h, U[:5] + h
+        else:
+            h = .5 * dt_dx * (_flux_x((U + d / 2)[:5]) - _flux_x(U[:5]))
+            wantL, wantR = U[:5] + h, U[:5] + d[:5] / 2 + h
+        assert np.allclose(L[:5], wantL, rtol=1e-12, atol=1e-14), (vx, L, wantL)
+        assert np.allclose(Rr[:5], wantR, rtol=1e-12, atol=1e-14), (vx, Rr, wantR)
+        assert L[5] == U[5] and Rr[5] == U[5]
+
+
+@pytest.mark.parametrize('limiter', ['minmod', 'van Leer', 'superbee'])
+@pytest.mark.parametrize('vx', [.35, -.5, 2., 0.])
+def test_plm_eig_against_eigendecomposition(limiter, vx):
+    """'plm eig' in 1D-shaped data (no transverse velocity or slope) against
+    numpy's eigendecomposition of the textbook 3x3 Jacobian: coefficients of
+    dUL, dUR in the right eigenvectors, sigma = phi(cL / cR) cR (the ratio
+    and the product do not depend on the eigenvectors' scaling), waves with
+    speed >= 0 give their slope to the left edge, <= 0 to the right edge, then
+    half a step of the edge fluxes F(UHalfR) - F(UHalfL) on both edges. Also
+    along y and z, with the velocity on that axis. vx = 0: fluid at rest,
+    the entropy wave's speed is exactly 0 and its slope reaches neither edge."""
+    rng = np.random.default_rng(17)
+    for trial in range(4):
+        W = [(1 + .3 * rng.random(), vx + (.1 * rng.random() if vx else 0.), .7 + .2 * rng.random()) for _ in range(3)]
+        cons3 = [np.array([r, r * u, p / .4 + .5 * r * u * u]) for r, u, p in W]
+        A = _jacobian_1d(cons3[1])
+        lam, V = np.linalg.eig(A)
+        lam, V = lam.real, V.real
+        if vx == 0:   # the exact eigenvalue u = 0, not numpy's rounding of it
+            assert np.sum(np.abs(lam) < 1e-12) == 1
+            lam = np.where(np.abs(lam) < 1e-12, 0., lam)
+        Vi = np.linalg.inv(V)
+        cL, cR = Vi @ (cons3[1] - cons3[0]), Vi @ (cons3[2] - cons3[1])
+        sig = np.array([_PHI[limiter](0. if cR[j] == 0 else cL[j] / cR[j]) * cR[j] for j in range(3)])
+        sL = V @ np.where(lam <= 0, 0, sig)
+        sR = V @ np.where(lam >= 0, 0, sig)
+        hL, hR = cons3[1] - .5 * sL, cons3[1] + .5 * sR
+        dt_dx = .41
+        f3 = lambda c: _flux_x(np.array([c[0], c[1], 0, 0, c[2]]))[[0, 1, 4]]   # noqa: E731
+        dF = f3(hR) - f3(hL)
+        wantL, wantR = hL + .5 * dt_dx * dF, hR + .5 * dt_dx * dF
+        for side in range(3):
+            def full(c):
+                U = np.zeros(6)
+                U[0], U[1 + side], U[4] = c
+                return U
+            L, Rr = _plm_eig(full(cons3[0]), full(cons3[1]), full(cons3[2]), side, dt_dx, limiter)
+            got = lambda X: X[[0, 1 + side, 4]]   # noqa: E731
+            assert np.allclose(got(L), wantL, rtol=1e-11, atol=1e-13), (side, got(L), wantL)
+            assert np.allclose(got(Rr), wantR, rtol=1e-11, atol=1e-13), (side, got(Rr), wantR)
+            others = [k for k in range(1, 4) if k != 1 + side]
+            assert np.all(L[others] == 0) and np.all(Rr[others] == 0)
+
+
+def test_plm_eig_reflection():
+    """mirroring the three cells along the axis (cells swapped, normal
+    momentum negated) mirrors the edge states: L' = mirror(R), R' = mirror(L);
+    with transverse velocities and slopes in every field"""
+    rng = np.random.default_rng(5)
+    for side in range(3):
+        for _ in range(5):
+            c = [_cons(1 + .5 * rng.random(), rng.uniform(-.6, .6, 3), .5 + rng.random()) for _ in range(3)]
+            m = lambda U: U * np.array([1, *(-1. if k == side else 1. for k in range(3)), 1, 1])   # noqa: E731
+            L, Rr = _plm_eig(c[0], c[1], c[2], side, .3, 'van Leer')
+            L2, R2 = _plm_eig(m(c[2]), m(c[1]), m(c[0]), side, .3, 'van Leer')
+            assert np.allclose(L2, m(Rr), rtol=1e-12, atol=1e-14) and np.allclose(R2, m(L), rtol=1e-12, atol=1e-14)
+
+
 def test_plm_cons_cases():
     c = np.array([1., 2, 3, 4, 5, 6])
     L, R = _plm(c, c, c)

@@ -14,7 +14,8 @@ Beside it on the same JSON line:
                 difference per conserved variable from the FP64 result
   fluxes        the same KH problem with the HLL, Rusanov, HLLC and Marquina
                 fluxes (plm cons / minmod)
-  reconstructions  the same KH problem, Roe flux, with 'plm prim' and 'plm eig'
+  reconstructions  the same KH problem, Roe flux, with 'plm prim', 'plm eig' and
+                'plm cons with flux'
   batched       512 independent 2D Euler four-quadrant grids (sides log-uniform
                 in [64, 512]), FP32, RK2-TVD, 300 steps: one persistent launch
                 per stage over all grids, hipGraph replay, autotuned launch
@@ -295,7 +296,7 @@ def main():
     ap.add_argument('--size', type=int, default=N)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-fp32', action='store_true', help='skip the FP32 KH leg')
-    ap.add_argument('--no-fluxes', action='store_true', help='skip the HLL / Rusanov / HLLC / Marquina and plm prim / plm eig KH legs')
+    ap.add_argument('--no-fluxes', action='store_true', help='skip the KH legs of the other fluxes and reconstructions')
     ap.add_argument('--no-batch', action='store_true', help='skip the batched leg')
     ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
     ap.add_argument('--no-3d', action='store_true', help='skip the 3D blast leg')
@@ -397,7 +398,7 @@ def main():
         # and the Roe flux with the other reconstructions ('plm prim', 'plm eig')
         out['fluxes'], out['reconstructions'] = {}, {}
         variants = [('fluxes', f, f, 'plm cons') for f in ('hll', 'rusanov', 'euler-hllc', 'marquina')]
-        variants += [('reconstructions', p, 'roe', p) for p in ('plm prim', 'plm eig')]
+        variants += [('reconstructions', p, 'roe', p) for p in ('plm prim', 'plm eig', 'plm cons with flux')]
         for leg, key, flux, plm in variants:
             sf = kh_solver(n, args.real, fast, args.rows, flux=flux, plm=plm)
             steps_f = max(20, args.steps // 5)

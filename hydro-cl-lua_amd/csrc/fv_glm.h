@@ -23,6 +23,7 @@ struct Eqn {
 	static constexpr int FLUXID = HYDRO_FLUX_ROE;
 	static constexpr bool PLM_PRIM = false;
 	static constexpr bool PLM_EIG = false;
+	static constexpr bool PLM_CONS_FLUX = false;
 	static constexpr int Q = 8;
 	static constexpr int NSTAT = 4;
 	static constexpr int NW = 8;

@@ -21,7 +21,7 @@ struct StageKey {
 	int nvel;       // Euler: active velocity components (dim or 3)
 	int flux;       // HYDRO_FLUX_*
 	int roeFix;     // Roe: Harten's entropy fix
-	int plmKind;    // HYDRO_PLM_* of the plm kernels ('plm cons', 'plm prim', 'plm eig')
+	int plmKind;    // HYDRO_PLM_* of the plm kernels
 };
 
 // Limiters with a compile-time form; the rest use the runtime switch.

@@ -86,6 +86,30 @@ __device__ __forceinline__ void shift_right(const Cell<R, E> &in, Cell<R, E> &ou
 	for (int f = 0; f < Cell<R, E>::NV; ++f) out.v[f] = from_right(in.v[f]);
 }
 
+// The half step of 'plm cons with flux' and 'plm eig' (plm.cl:169-188,
+// 410-428): both edge states hL, hR add (dt/dx)/2 (F(hR) - F(hL)).
+template <typename R, class E, int S>
+__device__ __forceinline__ void plm_half_step(const typename E::Par &par, const Cell<R, E> &U, const R *hL,
+	const R *hR, R dt_dx, Cell<R, E> &L, Cell<R, E> &Rr)
+{
+	constexpr int Q = E::Q;
+	R FL[Q], FR[Q];
+	E::template flux<S>(par, hL, E::prim(par, hL), FL);
+	E::template flux<S>(par, hR, E::prim(par, hR), FR);
+	const R h = R(.5) * dt_dx;
+#pragma unroll
+	for (int j = 0; j < Cell<R, E>::NV; ++j) {
+		if (j < Q) {
+			const R dF = FR[j] - FL[j];
+			L.v[j] = hL[j] + h * dF;
+			Rr.v[j] = hR[j] + h * dF;
+		} else {
+			L.v[j] = U.v[j];
+			Rr.v[j] = U.v[j];
+		}
+	}
+}
+
 // ------------------------------------------------------------- plm cons
 // plm.cl:27-91: one division per field (numerator / denominator chosen by the
 // sign of dUC, as the reference's two branches do). S: the axis of the
@@ -127,21 +151,22 @@ __device__ __forceinline__ void plm_cons(const typename E::Par &par, int lim, co
 			hL[j] = U.v[j] - R(.5) * sL[j];
 			hR[j] = U.v[j] + R(.5) * sR[j];
 		}
-		R FL[Q], FR[Q];
-		E::template flux<S>(par, hL, E::prim(par, hL), FL);
-		E::template flux<S>(par, hR, E::prim(par, hR), FR);
-		const R h = R(.5) * dt_dx;
+		plm_half_step<R, E, S>(par, U, hL, hR, dt_dx, L, Rr);
+		return;
+	} else if constexpr (E::PLM_CONS_FLUX) {
+		// 'plm cons with flux' (plm.cl:95-188): per field r = dUL / dUR,
+		// sigma = phi(r) dUR, edges U -+ sigma / 2, then the half step
+		constexpr int Q = E::Q;
+		R hL[Q], hR[Q];
 #pragma unroll
-		for (int j = 0; j < Cell<R, E>::NV; ++j) {
-			if (j < Q) {
-				const R dF = FR[j] - FL[j];
-				L.v[j] = hL[j] + h * dF;
-				Rr.v[j] = hR[j] + h * dF;
-			} else {
-				L.v[j] = U.v[j];
-				Rr.v[j] = U.v[j];
-			}
+		for (int j = 0; j < Q; ++j) {
+			const R dUL = U.v[j] - Um.v[j];
+			const R dUR = Up.v[j] - U.v[j];
+			const R sigma = limited<SLIM, E::FASTM>(lim, dUL, dUR);
+			hL[j] = U.v[j] - R(.5) * sigma;
+			hR[j] = U.v[j] + R(.5) * sigma;
 		}
+		plm_half_step<R, E, S>(par, U, hL, hR, dt_dx, L, Rr);
 		return;
 	} else if constexpr (E::PLM_PRIM) {
 		// 'plm prim' (plm.cl:191-254): the same slopes of the primitives

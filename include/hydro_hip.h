@@ -64,12 +64,14 @@ enum {
 	HYDRO_NUM_LIMITERS
 };
 
-/* Reconstruction (solver.usePLM): none, or 'plm cons' (hydro/solver/plm.cl:27-91). */
+/* Reconstruction (solver.usePLM): none, 'plm cons' (hydro/solver/plm.cl:27-91),
+ * or one of the Euler-only forms below. */
 enum {
 	HYDRO_PLM_NONE = 0,
 	HYDRO_PLM_CONS = 1,
 	HYDRO_PLM_PRIM = 2,   /* Euler, Roe flux (plm.cl:191-254) */
-	HYDRO_PLM_EIG = 3     /* Euler, Roe flux (plm.cl:256-432) */
+	HYDRO_PLM_EIG = 3,    /* Euler, Roe flux (plm.cl:256-432) */
+	HYDRO_PLM_CONS_FLUX = 4   /* 'plm cons with flux': Euler, Roe flux (plm.cl:95-188) */
 };
 
 /* Explicit integrators, hydro/int/fe.lua and the Shu-Osher tables of

@@ -801,6 +801,37 @@ void oracle_plm_eig(const oracle_params *p, int lim, const double *UL, const dou
 	}
 }
 
+void oracle_plm_cons_flux(const oracle_params *p, int lim, const double *UL, const double *U, const double *UR,
+	int s, double dt_dx, double *L, double *R)
+{
+	/* plm.cl:95-188 ('plm cons with flux'), Euler, Cartesian: per field
+	 * r = dUL / dUR (0 when dUR == 0), sigma = phi(r) dUR, UHalfL = U - sigma/2,
+	 * UHalfR = U + sigma/2; both edges then add (dt/dx)/2 (F(UHalfR) - F(UHalfL)) */
+	real UHalfL[6], UHalfR[6];
+	for (int j = 0; j < 6; ++j) UHalfL[j] = UHalfR[j] = U[j];
+	for (int j = 0; j < 5; ++j) {
+		real dUL = U[j] - UL[j];
+		real dUR = UR[j] - U[j];
+		real r = dUR == 0 ? 0 : (dUL / dUR);
+		real phi = oracle_limiter(lim, r);
+		real sigma = phi * dUR;
+		UHalfL[j] -= .5 * sigma;
+		UHalfR[j] += .5 * sigma;
+	}
+	real FHalfL[6], FHalfR[6];
+	oracle_euler_flux_from_cons(p, UHalfL, s, FHalfL);
+	oracle_euler_flux_from_cons(p, UHalfR, s, FHalfR);
+	for (int j = 0; j < 6; ++j) {
+		L[j] = UHalfL[j];
+		R[j] = UHalfR[j];
+	}
+	for (int j = 0; j < 5; ++j) {
+		real dF = FHalfR[j] - FHalfL[j];
+		L[j] += .5 * dt_dx * dF;
+		R[j] += .5 * dt_dx * dF;
+	}
+}
+
 void oracle_plm_cons(int lim, int nint, int S, const double *UL, const double *U, const double *UR,
 	double *L, double *R)
 {
@@ -912,8 +943,8 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 	if (cfg->flux < HYDRO_FLUX_ROE || cfg->flux > HYDRO_FLUX_MARQUINA) {
 		set_err("bad flux"); return NULL;
 	}
-	if ((cfg->plm == HYDRO_PLM_PRIM || cfg->plm == HYDRO_PLM_EIG) && cfg->eqn != HYDRO_EQN_EULER) {
-		set_err("plm prim and plm eig are built for the Euler equations"); return NULL;
+	if (cfg->plm > HYDRO_PLM_CONS && cfg->eqn != HYDRO_EQN_EULER) {
+		set_err("plm prim, plm eig and plm cons with flux are built for the Euler equations"); return NULL;
 	}
 	if (cfg->flux != HYDRO_FLUX_ROE && cfg->eqn != HYDRO_EQN_EULER) {
 		set_err("HLL, Rusanov, HLLC and Marquina take the Euler equations"); return NULL;
@@ -1123,7 +1154,11 @@ static void calc_lr(oracle_solver *s, const real *U, real dt) {
 	FOR_CELLS(s, 1, 1, {
 		for (int side = 0; side < dim; ++side) {
 			real *lr = s->ULR + (side + dim * index) * 2 * S;
-			if (s->cfg.plm == HYDRO_PLM_EIG)
+			if (s->cfg.plm == HYDRO_PLM_CONS_FLUX)
+				oracle_plm_cons_flux(&s->p, s->cfg.slope_limiter,
+					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
+					side, dt / s->dx[side], lr, lr + S);
+			else if (s->cfg.plm == HYDRO_PLM_EIG)
 				oracle_plm_eig(&s->p, s->cfg.slope_limiter,
 					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
 					side, dt / s->dx[side], lr, lr + S);

@@ -91,6 +91,10 @@ void oracle_plm_prim(const oracle_params *p, int lim, const double *UL, const do
  * dt_dx = dt / dx; L and R get 6 reals */
 void oracle_plm_eig(const oracle_params *p, int lim, const double *UL, const double *U, const double *UR,
 	int side, double dt_dx, double *L, double *R);
+/* 'plm cons with flux' edge states (plm.cl:95-188), Euler, along axis side
+ * with dt_dx = dt / dx; L and R get 6 reals */
+void oracle_plm_cons_flux(const oracle_params *p, int lim, const double *UL, const double *U, const double *UR,
+	int side, double dt_dx, double *L, double *R);
 void oracle_plm_cons(int slope_limiter, int nint, int nstates,
 	const double *UL, const double *U, const double *UR, double *L, double *R);
 

@@ -54,6 +54,7 @@ def test_create_rejects_plm_kinds_outside_euler_roe():
     flux; other pairings are refused before any device work"""
     L = _abi.lib()
     for kw in (dict(eqn='glm_maxwell', plm='plm eig'), dict(flux='hll', plm='plm eig'),
+               dict(eqn='glm_maxwell', plm='plm cons with flux'), dict(flux='marquina', plm='plm cons with flux'),
                dict(plm='plm eig', use_entropy_flux_fix=True), dict(eqn='glm_maxwell', plm='plm prim')):
         c = make_cfg(dim=2, size=(8, 8), **kw)
         h = ctypes.c_void_p()

@@ -562,6 +562,36 @@ def test_plm_eig_reflection():
             assert np.allclose(L2, m(Rr), rtol=1e-12, atol=1e-14) and np.allclose(R2, m(L), rtol=1e-12, atol=1e-14)
 
 
+def _plm_cons_flux(Um, U, Up, side=0, dt_dx=.3, limiter='minmod'):
+    L, Rr = np.zeros(6), np.zeros(6)
+    lib().oracle_plm_cons_flux(ctypes.byref(P), LIMITERS.index(limiter), _ptr(np.asarray(Um, float)),
+                               _ptr(np.asarray(U, float)), _ptr(np.asarray(Up, float)), side, dt_dx, _ptr(L), _ptr(Rr))
+    return L, Rr
+
+
+def test_plm_cons_with_flux_cases():
+    """'plm cons with flux': constant data gives back U; cons linear across
+    the three cells (minmod ratio 1) give the edges U -+ d/2 plus half a step
+    of the textbook flux difference, (dt/dx)/2 (F(U + d/2) - F(U - d/2)), on
+    both, along x, y and z; an extremum in every field gives back U (no slope,
+    no flux difference)"""
+    U = _cons(1.3, (.2, -.1, .05), .8)
+    L, Rr = _plm_cons_flux(U, U, U)
+    assert np.array_equal(L, U) and np.array_equal(Rr, U)
+    d = np.array([.02, -.01, .015, .005, .03, 0.])
+    dt_dx = .37
+    for side in range(3):
+        perm = [0, 1 + side, 1 + (side + 1) % 3, 1 + (side + 2) % 3, 4]
+        L, Rr = _plm_cons_flux(U - d, U, U + d, side, dt_dx)
+        Un, dn = U[perm], d[perm]   # the axis' momentum first, as _flux_x wants
+        h = .5 * dt_dx * (_flux_x(Un + dn / 2) - _flux_x(Un - dn / 2))
+        assert np.allclose(L[perm], Un - dn / 2 + h, rtol=1e-12, atol=1e-14), side
+        assert np.allclose(Rr[perm], Un + dn / 2 + h, rtol=1e-12, atol=1e-14), side
+        assert L[5] == U[5] and Rr[5] == U[5]
+    L, Rr = _plm_cons_flux(U - d, U, U - d)
+    assert np.allclose(L, U, rtol=1e-15) and np.allclose(Rr, U, rtol=1e-15)
+
+
 def test_plm_cons_cases():
     c = np.array([1., 2, 3, 4, 5, 6])
     L, R = _plm(c, c, c)

@@ -14,8 +14,8 @@ Beside it on the same JSON line:
                 difference per conserved variable from the FP64 result
   fluxes        the same KH problem with the HLL, Rusanov, HLLC and Marquina
                 fluxes (plm cons / minmod)
-  reconstructions  the same KH problem, Roe flux, with 'plm prim', 'plm eig' and
-                'plm cons with flux'
+  reconstructions  the same KH problem, Roe flux, with 'plm prim', 'plm eig',
+                'plm cons with flux', 'plm athena' and 'ppm'
   batched       512 independent 2D Euler four-quadrant grids (sides log-uniform
                 in [64, 512]), FP32, RK2-TVD, 300 steps: one persistent launch
                 per stage over all grids, hipGraph replay, autotuned launch
@@ -398,7 +398,8 @@ def main():
         # and the Roe flux with the other reconstructions ('plm prim', 'plm eig')
         out['fluxes'], out['reconstructions'] = {}, {}
         variants = [('fluxes', f, f, 'plm cons') for f in ('hll', 'rusanov', 'euler-hllc', 'marquina')]
-        variants += [('reconstructions', p, 'roe', p) for p in ('plm prim', 'plm eig', 'plm cons with flux')]
+        variants += [('reconstructions', p, 'roe', p)
+                     for p in ('plm prim', 'plm eig', 'plm cons with flux', 'plm athena', 'ppm')]
         for leg, key, flux, plm in variants:
             sf = kh_solver(n, args.real, fast, args.rows, flux=flux, plm=plm)
             steps_f = max(20, args.steps // 5)

@@ -22,8 +22,9 @@ namespace hydro {
 // FLUX: the interface flux the stage kernels build from this equation
 // (HYDRO_FLUX_ROE, _HLL, _RUSANOV, _HLLC, _MARQUINA, or FLUX_ROE_ENTROPY_FIX).
 // PLMK: the reconstruction of the PLM kernels, HYDRO_PLM_CONS, HYDRO_PLM_PRIM
-// (plm.cl:191-254), HYDRO_PLM_EIG (plm.cl:256-432) or HYDRO_PLM_CONS_FLUX
-// (plm.cl:95-188).
+// (plm.cl:191-254), HYDRO_PLM_EIG (plm.cl:256-432), HYDRO_PLM_CONS_FLUX
+// (plm.cl:95-188), HYDRO_PLM_ATHENA (plm.cl:782-879) or HYDRO_PLM_PPM
+// (plm.cl:882-944).
 template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE, int PLMK = HYDRO_PLM_CONS>
 struct EulerT {
 	template <typename R>
@@ -34,6 +35,8 @@ struct EulerT {
 		static constexpr bool PLM_PRIM = PLMK == HYDRO_PLM_PRIM;
 		static constexpr bool PLM_EIG = PLMK == HYDRO_PLM_EIG;
 		static constexpr bool PLM_CONS_FLUX = PLMK == HYDRO_PLM_CONS_FLUX;
+		static constexpr bool PLM_ATHENA = PLMK == HYDRO_PLM_ATHENA;
+		static constexpr bool PLM_PPM = PLMK == HYDRO_PLM_PPM;
 		static constexpr int Q = 5;       // numIntStates (euler.lua:14)
 		static constexpr int NSTAT = 0;   // no static fields on the device
 		static constexpr int NW = 5;      // numWaves (euler.lua:13)

@@ -24,6 +24,8 @@ struct Eqn {
 	static constexpr bool PLM_PRIM = false;
 	static constexpr bool PLM_EIG = false;
 	static constexpr bool PLM_CONS_FLUX = false;
+	static constexpr bool PLM_ATHENA = false;
+	static constexpr bool PLM_PPM = false;
 	static constexpr int Q = 8;
 	static constexpr int NSTAT = 4;
 	static constexpr int NW = 8;

@@ -168,6 +168,86 @@ __device__ __forceinline__ void plm_cons(const typename E::Par &par, int lim, co
 		}
 		plm_half_step<R, E, S>(par, U, hL, hR, dt_dx, L, Rr);
 		return;
+	} else if constexpr (E::PLM_ATHENA) {
+		// 'plm athena' (plm.cl:782-879; the later branch of that name is never
+		// reached): differences of the primitives put through the cell's cons
+		// eigensystem as they are, the Athena slope (the harmonic mean dWG
+		// beside left, right and center), back, each edge value clamped
+		// between the cell and its neighbour and the other edge moved to keep
+		// their sum; the cell's left edge takes the right value's cons and the
+		// right edge the left value's, as the reference stores them
+		using M = typename E::M;
+		const auto W = E::prim(par, U.v), Wm = E::prim(par, Um.v), Wp = E::prim(par, Up.v);
+		const auto e = E::eigen_cell(par, U.v, W);
+		const R w0[5] = {W.rho, W.v[0], W.v[1], W.v[2], W.P};
+		const R wm[5] = {Wm.rho, Wm.v[0], Wm.v[1], Wm.v[2], Wm.P};
+		const R wp[5] = {Wp.rho, Wp.v[0], Wp.v[1], Wp.v[2], Wp.P};
+		R dWL[5], dWR[5], dWC[5], dWG[5];
+#pragma unroll
+		for (int j = 0; j < 5; ++j) {
+			dWL[j] = w0[j] - wm[j];
+			dWR[j] = wp[j] - w0[j];
+			dWC[j] = R(.5) * (wp[j] - wm[j]);
+			dWG[j] = dWL[j] * dWR[j] <= R(0) ? R(0) : M::div(R(2) * dWL[j] * dWR[j], dWL[j] + dWR[j]);
+		}
+		R dal[5], dar[5], dac[5], dag[5];
+		E::template left<S>(par, e, dWL, dal);
+		E::template left<S>(par, e, dWR, dar);
+		E::template left<S>(par, e, dWC, dac);
+		E::template left<S>(par, e, dWG, dag);
+		R da[5];
+#pragma unroll
+		for (int j = 0; j < 5; ++j) {
+			da[j] = R(0);
+			if (dal[j] * dar[j] > R(0)) {
+				const R s1 = fmin(fabs(dal[j]), fabs(dar[j]));
+				const R s2 = fmin(fabs(dac[j]), fabs(dag[j]));
+				// OpenCL sign(): +-1, or the zero itself
+				const R sg = dac[j] > R(0) ? R(1) : dac[j] < R(0) ? R(-1) : dac[j];
+				da[j] = sg * fmin(R(2) * s1, s2);
+			}
+		}
+		R dWm[5];
+		E::template right<S>(par, e, da, dWm);
+		R wl[5], wr[5];
+#pragma unroll
+		for (int j = 0; j < 5; ++j) {
+			R lv = w0[j] - R(.5) * dWm[j];
+			R rv = w0[j] + R(.5) * dWm[j];
+			const R C = rv + lv;
+			lv = fmax(fmin(w0[j], wm[j]), lv);
+			lv = fmin(fmax(w0[j], wm[j]), lv);
+			rv = C - lv;
+			rv = fmax(fmin(w0[j], wp[j]), rv);
+			rv = fmin(fmax(w0[j], wp[j]), rv);
+			lv = C - rv;
+			wl[j] = lv;
+			wr[j] = rv;
+		}
+		E::cons_from_prim(par, wr, L.v);
+		E::cons_from_prim(par, wl, Rr.v);
+		return;
+	} else if constexpr (E::PLM_PPM) {
+		// 'ppm' (plm.cl:882-944): the kappa = 1/3 edges of the three-cell
+		// stencil scaled by phi(dUL / dUR); the second neighbours' differences
+		// it takes feed no output
+		using M = typename E::M;
+		constexpr R kappa = R(1) / R(3);
+#pragma unroll
+		for (int j = 0; j < Cell<R, E>::NV; ++j) {
+			if (j < E::Q) {
+				const R dUL = U.v[j] - Um.v[j];
+				const R dUR = Up.v[j] - U.v[j];
+				const R r = dUR == R(0) ? R(0) : (E::FASTM ? M::div(dUL, dUR) : dUL / dUR);
+				const R phi = limiter<SLIM>(lim, r);
+				Rr.v[j] = U.v[j] + R(.25) * phi * ((R(1) - kappa) * dUL + (R(1) + kappa) * dUR);
+				L.v[j] = U.v[j] - R(.25) * phi * ((R(1) - kappa) * dUR + (R(1) + kappa) * dUL);
+			} else {
+				L.v[j] = U.v[j];
+				Rr.v[j] = U.v[j];
+			}
+		}
+		return;
 	} else if constexpr (E::PLM_PRIM) {
 		// 'plm prim' (plm.cl:191-254): the same slopes of the primitives
 		// (rho, v, P), r = dWL / dWR always, back to cons at the two edges

@@ -71,7 +71,9 @@ enum {
 	HYDRO_PLM_CONS = 1,
 	HYDRO_PLM_PRIM = 2,   /* Euler, Roe flux (plm.cl:191-254) */
 	HYDRO_PLM_EIG = 3,    /* Euler, Roe flux (plm.cl:256-432) */
-	HYDRO_PLM_CONS_FLUX = 4   /* 'plm cons with flux': Euler, Roe flux (plm.cl:95-188) */
+	HYDRO_PLM_CONS_FLUX = 4,  /* 'plm cons with flux': Euler, Roe flux (plm.cl:95-188) */
+	HYDRO_PLM_ATHENA = 5, /* 'plm athena': Euler, Roe flux (plm.cl:782-879) */
+	HYDRO_PLM_PPM = 6     /* 'ppm': Euler, Roe flux (plm.cl:882-944) */
 };
 
 /* Explicit integrators, hydro/int/fe.lua and the Shu-Osher tables of

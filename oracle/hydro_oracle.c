@@ -832,6 +832,89 @@ void oracle_plm_cons_flux(const oracle_params *p, int lim, const double *UL, con
 	}
 }
 
+static real ocl_sign(real x) {
+	/* OpenCL sign(): 1 for x > 0, -1 for x < 0, the zero itself for +-0, 0 for NaN */
+	if (x > 0) return 1.;
+	if (x < 0) return -1.;
+	return x == x ? x : 0.;
+}
+
+void oracle_plm_athena(const oracle_params *p, const double *UL, const double *U, const double *UR,
+	int s, double *L, double *R)
+{
+	/* plm.cl:782-879 ('plm athena'; a second branch of the same name at
+	 * plm.cl:950 is never reached), Euler, Cartesian. The differences of the
+	 * primitives go through the cons eigensystem of the cell as they are
+	 * (the reference's own TODO asks whether it should be dA/dW). OpenCL
+	 * min/max on reals are fmin/fmax here (they differ only for NaN and in
+	 * the sign of a zero tie). */
+	real eig[10];
+	oracle_euler_eigen_for_cell(p, U, eig);
+	real W[6], WL[6], WR[6];
+	oracle_euler_prim_from_cons(p, U, W);
+	oracle_euler_prim_from_cons(p, UL, WL);
+	oracle_euler_prim_from_cons(p, UR, WR);
+	real dWL[6], dWR[6], dWC[6], dWG[6];
+	for (int j = 0; j < 5; ++j) {
+		dWL[j] = W[j] - WL[j];
+		dWR[j] = WR[j] - W[j];
+		dWC[j] = .5 * (WR[j] - WL[j]);
+		dWG[j] = (dWL[j] * dWR[j]) <= 0. ? 0. : (2. * dWL[j] * dWR[j] / (dWL[j] + dWR[j]));
+	}
+	dWL[5] = dWR[5] = dWC[5] = dWG[5] = 0.;
+	real dal[5], dar[5], dac[5], dag[5];
+	oracle_euler_left(p, eig, dWL, s, dal);
+	oracle_euler_left(p, eig, dWR, s, dar);
+	oracle_euler_left(p, eig, dWC, s, dac);
+	oracle_euler_left(p, eig, dWG, s, dag);
+	real da[5];
+	for (int j = 0; j < 5; ++j) {
+		da[j] = 0;
+		if (dal[j] * dar[j] > 0) {
+			real lim_slope1 = fmin(fabs(dal[j]), fabs(dar[j]));
+			real lim_slope2 = fmin(fabs(dac[j]), fabs(dag[j]));
+			da[j] = ocl_sign(dac[j]) * fmin(2. * lim_slope1, lim_slope2);
+		}
+	}
+	real dWm[6];
+	oracle_euler_right(p, eig, da, s, dWm);
+	real Wlv[6], Wrv[6];
+	for (int j = 0; j < 5; ++j) {
+		Wlv[j] = W[j] - .5 * dWm[j];
+		Wrv[j] = W[j] + .5 * dWm[j];
+		real C = Wrv[j] + Wlv[j];
+		Wlv[j] = fmax(fmin(W[j], WL[j]), Wlv[j]);
+		Wlv[j] = fmin(fmax(W[j], WL[j]), Wlv[j]);
+		Wrv[j] = C - Wlv[j];
+		Wrv[j] = fmax(fmin(W[j], WR[j]), Wrv[j]);
+		Wrv[j] = fmin(fmax(W[j], WR[j]), Wrv[j]);
+		Wlv[j] = C - Wrv[j];
+	}
+	Wlv[5] = Wrv[5] = W[5];
+	/* the cell's left edge takes Wrv, its right edge Wlv, as stored there */
+	oracle_euler_cons_from_prim(p, Wrv, L);
+	oracle_euler_cons_from_prim(p, Wlv, R);
+}
+
+void oracle_plm_ppm(int lim, int nint, int S, const double *UL, const double *U, const double *UR,
+	double *L, double *R)
+{
+	/* plm.cl:882-944 ('ppm'): with kappa = 1/3 and phi = phi(dUL / dUR),
+	 * uL_{i+1/2} = U + phi/4 ((1 - kappa) dUL + (1 + kappa) dUR) is the right
+	 * edge, uR_{i-1/2} = U - phi/4 ((1 - kappa) dUR + (1 + kappa) dUL) the left;
+	 * the differences with the second neighbours feed no output */
+	real const kappa = 1. / 3.;
+	for (int j = 0; j < S; ++j) L[j] = R[j] = U[j];
+	for (int j = 0; j < nint; ++j) {
+		real dUL = U[j] - UL[j];
+		real dUR = UR[j] - U[j];
+		real r = dUR == 0 ? 0 : dUL / dUR;
+		real phi = oracle_limiter(lim, r);
+		R[j] += .25 * phi * ((1. - kappa) * dUL + (1. + kappa) * dUR);
+		L[j] -= .25 * phi * ((1. - kappa) * dUR + (1. + kappa) * dUL);
+	}
+}
+
 void oracle_plm_cons(int lim, int nint, int S, const double *UL, const double *U, const double *UR,
 	double *L, double *R)
 {
@@ -944,7 +1027,7 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 		set_err("bad flux"); return NULL;
 	}
 	if (cfg->plm > HYDRO_PLM_CONS && cfg->eqn != HYDRO_EQN_EULER) {
-		set_err("plm prim, plm eig and plm cons with flux are built for the Euler equations"); return NULL;
+		set_err("the reconstructions other than plm cons are built for the Euler equations"); return NULL;
 	}
 	if (cfg->flux != HYDRO_FLUX_ROE && cfg->eqn != HYDRO_EQN_EULER) {
 		set_err("HLL, Rusanov, HLLC and Marquina take the Euler equations"); return NULL;
@@ -1154,7 +1237,15 @@ static void calc_lr(oracle_solver *s, const real *U, real dt) {
 	FOR_CELLS(s, 1, 1, {
 		for (int side = 0; side < dim; ++side) {
 			real *lr = s->ULR + (side + dim * index) * 2 * S;
-			if (s->cfg.plm == HYDRO_PLM_CONS_FLUX)
+			if (s->cfg.plm == HYDRO_PLM_ATHENA)
+				oracle_plm_athena(&s->p,
+					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
+					side, lr, lr + S);
+			else if (s->cfg.plm == HYDRO_PLM_PPM)
+				oracle_plm_ppm(s->cfg.slope_limiter, s->q, S,
+					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
+					lr, lr + S);
+			else if (s->cfg.plm == HYDRO_PLM_CONS_FLUX)
 				oracle_plm_cons_flux(&s->p, s->cfg.slope_limiter,
 					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
 					side, dt / s->dx[side], lr, lr + S);

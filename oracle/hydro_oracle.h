@@ -95,6 +95,13 @@ void oracle_plm_eig(const oracle_params *p, int lim, const double *UL, const dou
  * with dt_dx = dt / dx; L and R get 6 reals */
 void oracle_plm_cons_flux(const oracle_params *p, int lim, const double *UL, const double *U, const double *UR,
 	int side, double dt_dx, double *L, double *R);
+/* 'plm athena' edge states (plm.cl:782-879), Euler, along axis side; L and R
+ * get 6 reals */
+void oracle_plm_athena(const oracle_params *p, const double *UL, const double *U, const double *UR,
+	int side, double *L, double *R);
+/* 'ppm' edge states (plm.cl:882-944) */
+void oracle_plm_ppm(int slope_limiter, int nint, int nstates,
+	const double *UL, const double *U, const double *UR, double *L, double *R);
 void oracle_plm_cons(int slope_limiter, int nint, int nstates,
 	const double *UL, const double *U, const double *UR, double *L, double *R);
 

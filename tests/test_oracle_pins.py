@@ -592,6 +592,106 @@ def test_plm_cons_with_flux_cases():
     assert np.allclose(L, U, rtol=1e-15) and np.allclose(Rr, U, rtol=1e-15)
 
 
+def _plm_ppm(Um, U, Up, limiter='minmod'):
+    L, Rr = np.zeros(6), np.zeros(6)
+    lib().oracle_plm_ppm(LIMITERS.index(limiter), 5, 6, _ptr(np.asarray(Um, float)), _ptr(np.asarray(U, float)),
+                         _ptr(np.asarray(Up, float)), _ptr(L), _ptr(Rr))
+    return L, Rr
+
+
+def test_plm_ppm_cases():
+    """'ppm' (kappa = 1/3): constant data gives back U; linear data with any
+    limiter that is 1 at r = 1 gives the exact edges U -+ d/2; the cell
+    averages of a parabola with phi = 1 ('Lax-Wendroff') give the parabola's
+    exact edge values (the kappa = 1/3 scheme is third order); an extremum
+    with minmod gives back U; the non-integrated field is copied"""
+    c = np.array([1., 2, 3, 4, 5, 6])
+    L, Rr = _plm_ppm(c, c, c)
+    assert np.array_equal(L, c) and np.array_equal(Rr, c)
+    d = np.array([.3, -.2, .1, .05, .4, 0.])
+    for lim in ('minmod', 'van Leer', 'superbee', 'monotized central'):
+        L, Rr = _plm_ppm(c - d, c, c + d, lim)
+        assert np.allclose(L[:5], (c - d / 2)[:5], rtol=1e-15) and np.allclose(Rr[:5], (c + d / 2)[:5], rtol=1e-15)
+        assert L[5] == 6 and Rr[5] == 6
+    rng = np.random.default_rng(2)
+    a, b, q = rng.uniform(1, 2, 6), rng.uniform(-1, 1, 6), rng.uniform(-1, 1, 6)
+    avg = lambda x0: a + b * x0 + q * (x0 * x0 + 1 / 12)   # noqa: E731  mean of a + b x + q x^2 over [x0 - 1/2, x0 + 1/2]
+    L, Rr = _plm_ppm(avg(-1.), avg(0.), avg(1.), 'Lax-Wendroff')
+    edge = lambda x: a + b * x + q * x * x   # noqa: E731
+    assert np.allclose(L[:5], edge(-.5)[:5], rtol=1e-14) and np.allclose(Rr[:5], edge(.5)[:5], rtol=1e-14)
+    L, Rr = _plm_ppm(c - d, c, c - d)
+    assert np.array_equal(L, c) and np.array_equal(Rr, c)
+
+
+def _plm_athena(Um, U, Up, side=0):
+    L, Rr = np.zeros(6), np.zeros(6)
+    lib().oracle_plm_athena(ctypes.byref(P), _ptr(np.asarray(Um, float)), _ptr(np.asarray(U, float)),
+                            _ptr(np.asarray(Up, float)), side, _ptr(L), _ptr(Rr))
+    return L, Rr
+
+
+def test_plm_athena_closed_forms():
+    """'plm athena': constant data gives back U; primitives linear across the
+    three cells give back their slope (all four characteristic slopes agree,
+    the transforms are inverse), so the cell's left edge holds cons(W + dW/2)
+    and its right edge cons(W - dW/2), the order the reference stores them
+    in; an extremum in every primitive gives back U"""
+    U = _cons(1.3, (.2, -.1, .05), .8)
+    L, Rr = _plm_athena(U, U, U)
+    assert np.allclose(L, U, rtol=1e-15) and np.allclose(Rr, U, rtol=1e-15)
+    W0 = np.array([1.3, .2, -.1, .05, .8])
+    dW = np.array([.1, -.03, .02, .01, .05])
+    for side in range(3):
+        Um, U0, Up = (_textbook_cons(w[0], w[1:4], w[4]) for w in (W0 - dW, W0, W0 + dW))
+        L, Rr = _plm_athena(Um, U0, Up, side)
+        WL, WR = W0 - dW / 2, W0 + dW / 2
+        assert np.allclose(L, _textbook_cons(WR[0], WR[1:4], WR[4]), rtol=1e-13, atol=1e-15), side
+        assert np.allclose(Rr, _textbook_cons(WL[0], WL[1:4], WL[4]), rtol=1e-13, atol=1e-15), side
+    Wext = W0 + dW
+    Uext = _textbook_cons(Wext[0], Wext[1:4], Wext[4])
+    L, Rr = _plm_athena(U0, Uext, U0)
+    assert np.allclose(L, Uext, rtol=1e-14) and np.allclose(Rr, Uext, rtol=1e-14)
+
+
+@pytest.mark.parametrize('vx', [.35, -.5, 2.])
+def test_plm_athena_against_eigendecomposition(vx):
+    """'plm athena' in 1D-shaped data against numpy's eigendecomposition of
+    the textbook 3x3 cons Jacobian, applied to the (rho, v, P) differences as
+    the reference does: the Athena slope sign(c) min(2 min(|l|, |r|), |c|,
+    |g|) per wave when l r > 0 (g: the harmonic-mean difference; the slope
+    does not depend on the eigenvectors' scaling), back, then each edge
+    clamped between the cell and its neighbour with the other moved to keep
+    their sum; along x, y and z"""
+    rng = np.random.default_rng(23)
+    for trial in range(6):
+        W = [np.array([1 + .4 * rng.random(), vx + .2 * rng.random(), .7 + .3 * rng.random()]) for _ in range(3)]
+        c3 = lambda w: np.array([w[0], w[0] * w[1], w[2] / .4 + .5 * w[0] * w[1] ** 2])   # noqa: E731
+        lam, V = np.linalg.eig(_jacobian_1d(c3(W[1])))
+        V = V.real
+        Vi = np.linalg.inv(V)
+        dL, dR = W[1] - W[0], W[2] - W[1]
+        dC = .5 * (W[2] - W[0])
+        dG = np.where(dL * dR <= 0, 0., 2 * dL * dR / np.where(dL + dR == 0, 1, dL + dR))
+        al, ar, ac, ag = (Vi @ x for x in (dL, dR, dC, dG))
+        da = np.where(al * ar > 0, np.sign(ac) * np.minimum(2 * np.minimum(abs(al), abs(ar)),
+                                                            np.minimum(abs(ac), abs(ag))), 0.)
+        dWm = V @ da
+        lv, rv = W[1] - .5 * dWm, W[1] + .5 * dWm
+        C = lv + rv
+        lv = np.clip(lv, np.minimum(W[1], W[0]), np.maximum(W[1], W[0]))
+        rv = np.clip(C - lv, np.minimum(W[1], W[2]), np.maximum(W[1], W[2]))
+        lv = C - rv
+        for side in range(3):
+            def full(w):
+                U = np.zeros(6)
+                U[0], U[1 + side], U[4] = c3(w)
+                return U
+            L, Rr = _plm_athena(full(W[0]), full(W[1]), full(W[2]), side)
+            got = lambda X: X[[0, 1 + side, 4]]   # noqa: E731
+            assert np.allclose(got(L), c3(rv), rtol=1e-11, atol=1e-13), (trial, side)
+            assert np.allclose(got(Rr), c3(lv), rtol=1e-11, atol=1e-13), (trial, side)
+
+
 def test_plm_cons_cases():
     c = np.array([1., 2, 3, 4, 5, 6])
     L, R = _plm(c, c, c)

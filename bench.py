@@ -15,7 +15,8 @@ Beside it on the same JSON line:
   fluxes        the same KH problem with the HLL, Rusanov, HLLC and Marquina
                 fluxes (plm cons / minmod)
   reconstructions  the same KH problem, Roe flux, with 'plm prim', 'plm eig',
-                'plm cons with flux', 'plm athena' and 'ppm'
+                'plm cons with flux', 'plm athena', 'ppm', 'plm eig prim' and
+                'plm eig prim ref'
   batched       512 independent 2D Euler four-quadrant grids (sides log-uniform
                 in [64, 512]), FP32, RK2-TVD, 300 steps: one persistent launch
                 per stage over all grids, hipGraph replay, autotuned launch
@@ -399,7 +400,8 @@ def main():
         out['fluxes'], out['reconstructions'] = {}, {}
         variants = [('fluxes', f, f, 'plm cons') for f in ('hll', 'rusanov', 'euler-hllc', 'marquina')]
         variants += [('reconstructions', p, 'roe', p)
-                     for p in ('plm prim', 'plm eig', 'plm cons with flux', 'plm athena', 'ppm')]
+                     for p in ('plm prim', 'plm eig', 'plm cons with flux', 'plm athena', 'ppm', 'plm eig prim',
+                               'plm eig prim ref')]
         for leg, key, flux, plm in variants:
             sf = kh_solver(n, args.real, fast, args.rows, flux=flux, plm=plm)
             steps_f = max(20, args.steps // 5)

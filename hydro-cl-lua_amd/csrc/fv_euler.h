@@ -23,8 +23,8 @@ namespace hydro {
 // (HYDRO_FLUX_ROE, _HLL, _RUSANOV, _HLLC, _MARQUINA, or FLUX_ROE_ENTROPY_FIX).
 // PLMK: the reconstruction of the PLM kernels, HYDRO_PLM_CONS, HYDRO_PLM_PRIM
 // (plm.cl:191-254), HYDRO_PLM_EIG (plm.cl:256-432), HYDRO_PLM_CONS_FLUX
-// (plm.cl:95-188), HYDRO_PLM_ATHENA (plm.cl:782-879) or HYDRO_PLM_PPM
-// (plm.cl:882-944).
+// (plm.cl:95-188), HYDRO_PLM_ATHENA (plm.cl:782-879), HYDRO_PLM_PPM
+// (plm.cl:882-944) or HYDRO_PLM_EIG_PRIM(_REF) (plm.cl:536-780).
 template <bool FAST, int NVEL, int FLUX = HYDRO_FLUX_ROE, int PLMK = HYDRO_PLM_CONS>
 struct EulerT {
 	template <typename R>
@@ -37,6 +37,8 @@ struct EulerT {
 		static constexpr bool PLM_CONS_FLUX = PLMK == HYDRO_PLM_CONS_FLUX;
 		static constexpr bool PLM_ATHENA = PLMK == HYDRO_PLM_ATHENA;
 		static constexpr bool PLM_PPM = PLMK == HYDRO_PLM_PPM;
+		static constexpr bool PLM_EIG_PRIM = PLMK == HYDRO_PLM_EIG_PRIM;
+		static constexpr bool PLM_EIG_PRIM_REF = PLMK == HYDRO_PLM_EIG_PRIM_REF;
 		static constexpr int Q = 5;       // numIntStates (euler.lua:14)
 		static constexpr int NSTAT = 0;   // no static fields on the device
 		static constexpr int NW = 5;      // numWaves (euler.lua:13)
@@ -72,6 +74,13 @@ struct EulerT {
 			T s = m[0] * m[0];
 			if constexpr (NVEL > 1) s = s + m[1] * m[1];
 			if constexpr (NVEL > 2) s = s + m[2] * m[2];
+			return s;
+		}
+		template <typename T>
+		__device__ static __forceinline__ T dot3(const T *a, const T *b) {
+			T s = a[0] * b[0];
+			if constexpr (NVEL > 1) s = s + a[1] * b[1];
+			if constexpr (NVEL > 2) s = s + a[2] * b[2];
 			return s;
 		}
 
@@ -174,6 +183,29 @@ struct EulerT {
 #pragma unroll
 			for (int i = 0; i < 3; ++i) U[1 + i] = i < NVEL ? W[1 + i] * W[0] : R(0);
 			U[4] = W[0] * (R(.5) * lenSq3(W + 1)) + M::div(W[4], p.g1);
+		}
+
+		// apply_dU_dW (euler.cl:179-195): the cons difference of the prim
+		// difference dW about the state WA; prims as (rho, v, P)
+		__device__ static __forceinline__ void apply_dU_dW(const Par &p, const R *WA, const R *dW, R *dU) {
+			dU[0] = dW[0];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) dU[1 + i] = i < NVEL ? WA[1 + i] * dW[0] + dW[1 + i] * WA[0] : R(0);
+			dU[4] = dW[0] * R(.5) * lenSq3(WA + 1) + WA[0] * dot3(dW + 1, WA + 1) + M::div(dW[4], p.g1);
+		}
+
+		// apply_dW_dU (euler.cl:201-223): the prim difference of the cons
+		// difference dU about WA; velocity and pressure 0 where dU's density
+		// is below rhoMin, as the reference tests it
+		__device__ static __forceinline__ void apply_dW_dU(const Par &p, const R *WA, const R *dU, R *dW) {
+			const bool low = dU[0] < p.rhoMin;
+			const R invRho = M::rcp(WA[0]);
+			const R rhoRatio = M::div(dU[0], WA[0]);
+			dW[0] = dU[0];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) dW[1 + i] = i < NVEL && !low ? dU[1 + i] * invRho - WA[1 + i] * rhoRatio : R(0);
+			const R P = p.g1 * (R(.5) * lenSq3(WA + 1) * dU[0] - dot3(dU + 1, WA + 1) + dU[4]);
+			dW[4] = low ? R(0) : P;
 		}
 
 		// eigen_forCell (euler.cl:315-341): one cell's own state, no guards

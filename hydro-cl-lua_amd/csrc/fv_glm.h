@@ -26,6 +26,8 @@ struct Eqn {
 	static constexpr bool PLM_CONS_FLUX = false;
 	static constexpr bool PLM_ATHENA = false;
 	static constexpr bool PLM_PPM = false;
+	static constexpr bool PLM_EIG_PRIM = false;
+	static constexpr bool PLM_EIG_PRIM_REF = false;
 	static constexpr int Q = 8;
 	static constexpr int NSTAT = 4;
 	static constexpr int NW = 8;

@@ -168,12 +168,13 @@ static int validate(const hydro_cfg *c) {
 		return fail("roe_entropy_fix_factor must be > 0");
 	if (c->flux_limiter < 0 || c->flux_limiter >= HYDRO_NUM_LIMITERS) return fail("flux_limiter %d out of range", c->flux_limiter);
 	if (c->slope_limiter < 0 || c->slope_limiter >= HYDRO_NUM_LIMITERS) return fail("slope_limiter %d out of range", c->slope_limiter);
-	if (c->plm < HYDRO_PLM_NONE || c->plm > HYDRO_PLM_PPM)
+	if (c->plm < HYDRO_PLM_NONE || c->plm > HYDRO_PLM_EIG_PRIM_REF)
 		return fail("plm %d not supported", c->plm);
 	if (c->plm > HYDRO_PLM_CONS && (c->eqn != HYDRO_EQN_EULER || c->flux != HYDRO_FLUX_ROE || c->roe_entropy_fix))
 		return fail("%s is built for the Euler equations with the Roe flux (no entropy fix)",
 			c->plm == HYDRO_PLM_PRIM ? "plm prim" : c->plm == HYDRO_PLM_EIG ? "plm eig"
-			: c->plm == HYDRO_PLM_CONS_FLUX ? "plm cons with flux" : c->plm == HYDRO_PLM_ATHENA ? "plm athena" : "ppm");
+			: c->plm == HYDRO_PLM_CONS_FLUX ? "plm cons with flux" : c->plm == HYDRO_PLM_ATHENA ? "plm athena"
+			: c->plm == HYDRO_PLM_PPM ? "ppm" : c->plm == HYDRO_PLM_EIG_PRIM ? "plm eig prim" : "plm eig prim ref");
 	// gridsolver.lua:119
 	if (c->plm && c->flux_limiter != HYDRO_LIMITER_DONOR_CELL)
 		return fail("are you sure you want to use flux and slope limiters at the same time?");

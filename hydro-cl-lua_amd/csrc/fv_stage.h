@@ -168,6 +168,78 @@ __device__ __forceinline__ void plm_cons(const typename E::Par &par, int lim, co
 		}
 		plm_half_step<R, E, S>(par, U, hL, hR, dt_dx, L, Rr);
 		return;
+	} else if constexpr (E::PLM_EIG_PRIM || E::PLM_EIG_PRIM_REF) {
+		// 'plm eig prim' / 'plm eig prim ref' (plm.cl:536-780): primitive
+		// differences mapped by dU/dW into the cell's characteristic
+		// variables, the doubled minmod-of-three slope, per-wave slopes for
+		// each edge (with the reference states of the 'ref' form), back by
+		// dW/dU, and the edge primitives to cons
+		const auto W = E::prim(par, U.v), Wm = E::prim(par, Um.v), Wp = E::prim(par, Up.v);
+		const auto e = E::eigen_cell(par, U.v, W);
+		const R w0[5] = {W.rho, W.v[0], W.v[1], W.v[2], W.P};
+		const R wm[5] = {Wm.rho, Wm.v[0], Wm.v[1], Wm.v[2], Wm.P};
+		const R wp[5] = {Wp.rho, Wp.v[0], Wp.v[1], Wp.v[2], Wp.P};
+		R dWL[5], dWR[5], dWC[5];
+#pragma unroll
+		for (int j = 0; j < 5; ++j) {
+			dWL[j] = w0[j] - wm[j];
+			dWR[j] = wp[j] - w0[j];
+			dWC[j] = R(.5) * (wp[j] - wm[j]);
+		}
+		R t[5], eL[5], eR[5], eC[5];
+		E::apply_dU_dW(par, w0, dWL, t);
+		E::template left<S>(par, e, t, eL);
+		E::apply_dU_dW(par, w0, dWR, t);
+		E::template left<S>(par, e, t, eR);
+		E::apply_dU_dW(par, w0, dWC, t);
+		E::template left<S>(par, e, t, eC);
+		R dM[5], lam[5];
+#pragma unroll
+		for (int j = 0; j < 5; ++j)
+			dM[j] = eL[j] * eR[j] < R(0) ? R(0)
+				: (eC[j] >= R(0) ? R(1) : R(-1)) * R(2) * fmin(fmin(fabs(eL[j]), fabs(eR[j])), fabs(eC[j]));
+		E::template waves<S>(par, e, lam);
+		R aL[5], aR[5], sL[5], sR[5], w2L[5], w2R[5];
+		if constexpr (E::PLM_EIG_PRIM) {
+#pragma unroll
+			for (int j = 0; j < 5; ++j) {
+				aL[j] = lam[j] < R(0) ? R(0) : dM[j] * R(.5) * (R(1) - lam[j] * dt_dx);
+				aR[j] = lam[j] > R(0) ? R(0) : dM[j] * R(.5) * (R(1) + lam[j] * dt_dx);
+			}
+			E::template right<S>(par, e, aL, t);
+			E::apply_dW_dU(par, w0, t, sL);
+			E::template right<S>(par, e, aR, t);
+			E::apply_dW_dU(par, w0, t, sR);
+#pragma unroll
+			for (int j = 0; j < 5; ++j) {
+				w2L[j] = w0[j] + sL[j];
+				w2R[j] = w0[j] - sR[j];
+			}
+		} else {
+			const R waveMin = fmin(R(0), lam[0]), waveMax = fmax(R(0), lam[4]);
+			R dWM[5];
+			E::template right<S>(par, e, dM, t);
+			E::apply_dW_dU(par, w0, t, dWM);
+#pragma unroll
+			for (int j = 0; j < 5; ++j) {
+				aL[j] = lam[j] < R(0) ? R(0) : dM[j] * dt_dx * (waveMax - lam[j]);
+				aR[j] = lam[j] > R(0) ? R(0) : dM[j] * dt_dx * (waveMin - lam[j]);
+			}
+			E::template right<S>(par, e, aL, t);
+			E::apply_dW_dU(par, w0, t, sL);
+			E::template right<S>(par, e, aR, t);
+			E::apply_dW_dU(par, w0, t, sR);
+#pragma unroll
+			for (int j = 0; j < 5; ++j) {
+				const R wLRef = w0[j] + R(.5) * (R(1) - dt_dx * waveMax) * dWM[j];
+				const R wRRef = w0[j] - R(.5) * (R(1) + dt_dx * waveMin) * dWM[j];
+				w2R[j] = wRRef + R(.5) * sR[j];
+				w2L[j] = wLRef + R(.5) * sL[j];
+			}
+		}
+		E::cons_from_prim(par, w2L, L.v);
+		E::cons_from_prim(par, w2R, Rr.v);
+		return;
 	} else if constexpr (E::PLM_ATHENA) {
 		// 'plm athena' (plm.cl:782-879; the later branch of that name is never
 		// reached): differences of the primitives put through the cell's cons

@@ -29,7 +29,8 @@ INTEGRATOR_STAGES = [1, 2, 2, 2, 3, 4, 4, 2, 2, 3, 4, 4]
 BOUNDARIES = ['periodic', 'mirror', 'freeflow', 'fixed']
 EQNS = ['euler', 'glm_maxwell']
 FLUXES = ['roe', 'hll', 'rusanov', 'euler-hllc', 'marquina']
-PLMS = [None, 'plm cons', 'plm prim', 'plm eig', 'plm cons with flux', 'plm athena', 'ppm']
+PLMS = [None, 'plm cons', 'plm prim', 'plm eig', 'plm cons with flux', 'plm athena', 'ppm',
+        'plm eig prim', 'plm eig prim ref']
 # plm.cl:10-24: the edge states are the cell's own, the flux of no
 # reconstruction (which the flux limiter check of gridsolver.lua:119 keeps at
 # 'donor cell'), so it runs as plm None

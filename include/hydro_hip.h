@@ -73,7 +73,9 @@ enum {
 	HYDRO_PLM_EIG = 3,    /* Euler, Roe flux (plm.cl:256-432) */
 	HYDRO_PLM_CONS_FLUX = 4,  /* 'plm cons with flux': Euler, Roe flux (plm.cl:95-188) */
 	HYDRO_PLM_ATHENA = 5, /* 'plm athena': Euler, Roe flux (plm.cl:782-879) */
-	HYDRO_PLM_PPM = 6     /* 'ppm': Euler, Roe flux (plm.cl:882-944) */
+	HYDRO_PLM_PPM = 6,    /* 'ppm': Euler, Roe flux (plm.cl:882-944) */
+	HYDRO_PLM_EIG_PRIM = 7,      /* 'plm eig prim': Euler, Roe flux (plm.cl:536-701) */
+	HYDRO_PLM_EIG_PRIM_REF = 8   /* 'plm eig prim ref': Euler, Roe flux (plm.cl:536-615, 703-776) */
 };
 
 /* Explicit integrators, hydro/int/fe.lua and the Shu-Osher tables of

@@ -896,6 +896,106 @@ void oracle_plm_athena(const oracle_params *p, const double *UL, const double *U
 	oracle_euler_cons_from_prim(p, Wlv, R);
 }
 
+static void euler_apply_dU_dW(const oracle_params *p, const real *WA, const real *W, real *U) {
+	/* euler.cl:179-195, Cartesian (vL = v); prims as (rho, v, P, ePot) */
+	U[0] = W[0];
+	for (int i = 0; i < 3; ++i) U[1 + i] = WA[1 + i] * W[0] + W[1 + i] * WA[0];
+	U[4] = W[0] * .5 * dot3(WA + 1, WA + 1) + WA[0] * dot3(W + 1, WA + 1) + W[4] / (p->gamma - 1.);
+	U[5] = W[5];
+}
+
+static void euler_apply_dW_dU(const oracle_params *p, const real *WA, const real *U, real *W) {
+	/* euler.cl:201-223, Cartesian: the rhoMin test is on U's (the
+	 * difference's) density */
+	W[0] = U[0];
+	if (U[0] < p->rho_min) {
+		W[1] = W[2] = W[3] = 0.;
+		W[4] = 0.;
+	} else {
+		for (int i = 0; i < 3; ++i) W[1 + i] = U[1 + i] * (1. / WA[0]) - WA[1 + i] * (U[0] / WA[0]);
+		W[4] = (p->gamma - 1.) * (.5 * dot3(WA + 1, WA + 1) * U[0] - dot3(U + 1, WA + 1) + U[4]);
+	}
+	W[5] = U[5];
+}
+
+void oracle_plm_eig_prim(const oracle_params *p, int ref, const double *UL, const double *U, const double *UR,
+	int s, double dt_dx, double *L, double *R)
+{
+	/* plm.cl:536-780 ('plm eig prim', ref = 0; 'plm eig prim ref', ref = 1),
+	 * Euler, Cartesian */
+	real W[6], WL[6], WR[6];
+	oracle_euler_prim_from_cons(p, U, W);
+	oracle_euler_prim_from_cons(p, UL, WL);
+	oracle_euler_prim_from_cons(p, UR, WR);
+	real dWL[6], dWR[6], dWC[6];
+	for (int j = 0; j < 5; ++j) {
+		dWL[j] = W[j] - WL[j];
+		dWR[j] = WR[j] - W[j];
+		dWC[j] = .5 * (WR[j] - WL[j]);
+	}
+	dWL[5] = dWR[5] = dWC[5] = 0.;
+	real eig[10];
+	oracle_euler_eigen_for_cell(p, U, eig);
+	real tmp[6], dWLEig[5], dWREig[5], dWCEig[5];
+	euler_apply_dU_dW(p, W, dWL, tmp);
+	oracle_euler_left(p, eig, tmp, s, dWLEig);
+	euler_apply_dU_dW(p, W, dWR, tmp);
+	oracle_euler_left(p, eig, tmp, s, dWREig);
+	euler_apply_dU_dW(p, W, dWC, tmp);
+	oracle_euler_left(p, eig, tmp, s, dWCEig);
+	real dWMEig[5];
+	for (int j = 0; j < 5; ++j)
+		dWMEig[j] = dWLEig[j] * dWREig[j] < 0. ? 0.
+			: ((dWCEig[j] >= 0. ? 1. : -1.) * 2. * fmin(fmin(fabs(dWLEig[j]), fabs(dWREig[j])), fabs(dWCEig[j])));
+	real wave[5];
+	oracle_euler_waves(p, eig, s, wave);
+	real aL[5], aR[5], sL[6], sR[6], W2L[6], W2R[6];
+	if (!ref) {
+		for (int j = 0; j < 5; ++j) {
+			aL[j] = wave[j] < 0 ? 0 : dWMEig[j] * .5 * (1. - wave[j] * dt_dx);
+			aR[j] = wave[j] > 0 ? 0 : dWMEig[j] * .5 * (1. + wave[j] * dt_dx);
+		}
+		tmp[5] = 0;
+		oracle_euler_right(p, eig, aL, s, tmp);
+		euler_apply_dW_dU(p, W, tmp, sL);
+		oracle_euler_right(p, eig, aR, s, tmp);
+		euler_apply_dW_dU(p, W, tmp, sR);
+		for (int j = 0; j < 5; ++j) {
+			W2L[j] = W[j] + sL[j];
+			W2R[j] = W[j] - sR[j];
+		}
+	} else {
+		/* eigenWaveCodeMinMax (eqn.lua:1108-1120): the first and last waves */
+		real waveMin = wave[0], waveMax = wave[4];
+		waveMin = fmin(0., waveMin);
+		waveMax = fmax(0., waveMax);
+		real dWM[6];
+		tmp[5] = 0;
+		oracle_euler_right(p, eig, dWMEig, s, tmp);
+		euler_apply_dW_dU(p, W, tmp, dWM);
+		real WLRef[5], WRRef[5];
+		for (int j = 0; j < 5; ++j) {
+			WLRef[j] = W[j] + .5 * (1. - dt_dx * waveMax) * dWM[j];
+			WRRef[j] = W[j] - .5 * (1. + dt_dx * waveMin) * dWM[j];
+		}
+		for (int j = 0; j < 5; ++j) {
+			aL[j] = wave[j] < 0 ? 0 : (dWMEig[j] * dt_dx * (waveMax - wave[j]));
+			aR[j] = wave[j] > 0 ? 0 : (dWMEig[j] * dt_dx * (waveMin - wave[j]));
+		}
+		oracle_euler_right(p, eig, aL, s, tmp);
+		euler_apply_dW_dU(p, W, tmp, sL);
+		oracle_euler_right(p, eig, aR, s, tmp);
+		euler_apply_dW_dU(p, W, tmp, sR);
+		for (int j = 0; j < 5; ++j) {
+			W2R[j] = WRRef[j] + .5 * sR[j];
+			W2L[j] = WLRef[j] + .5 * sL[j];
+		}
+	}
+	W2L[5] = W2R[5] = W[5];
+	oracle_euler_cons_from_prim(p, W2L, L);
+	oracle_euler_cons_from_prim(p, W2R, R);
+}
+
 void oracle_plm_ppm(int lim, int nint, int S, const double *UL, const double *U, const double *UR,
 	double *L, double *R)
 {
@@ -1237,7 +1337,11 @@ static void calc_lr(oracle_solver *s, const real *U, real dt) {
 	FOR_CELLS(s, 1, 1, {
 		for (int side = 0; side < dim; ++side) {
 			real *lr = s->ULR + (side + dim * index) * 2 * S;
-			if (s->cfg.plm == HYDRO_PLM_ATHENA)
+			if (s->cfg.plm == HYDRO_PLM_EIG_PRIM || s->cfg.plm == HYDRO_PLM_EIG_PRIM_REF)
+				oracle_plm_eig_prim(&s->p, s->cfg.plm == HYDRO_PLM_EIG_PRIM_REF,
+					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
+					side, dt / s->dx[side], lr, lr + S);
+			else if (s->cfg.plm == HYDRO_PLM_ATHENA)
 				oracle_plm_athena(&s->p,
 					U + (index - s->step[side]) * S, U + index * S, U + (index + s->step[side]) * S,
 					side, lr, lr + S);

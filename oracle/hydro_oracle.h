@@ -99,6 +99,10 @@ void oracle_plm_cons_flux(const oracle_params *p, int lim, const double *UL, con
  * get 6 reals */
 void oracle_plm_athena(const oracle_params *p, const double *UL, const double *U, const double *UR,
 	int side, double *L, double *R);
+/* 'plm eig prim' (ref = 0) and 'plm eig prim ref' (ref = 1) edge states
+ * (plm.cl:536-780), Euler, along axis side with dt_dx = dt / dx */
+void oracle_plm_eig_prim(const oracle_params *p, int ref, const double *UL, const double *U, const double *UR,
+	int side, double dt_dx, double *L, double *R);
 /* 'ppm' edge states (plm.cl:882-944) */
 void oracle_plm_ppm(int slope_limiter, int nint, int nstates,
 	const double *UL, const double *U, const double *UR, double *L, double *R);

@@ -82,6 +82,7 @@ def lib():
         L.oracle_plm_cons_flux.argtypes = L.oracle_plm_eig.argtypes
         L.oracle_plm_athena.argtypes = [pp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]
         L.oracle_plm_ppm.argtypes = L.oracle_plm_cons.argtypes
+        L.oracle_plm_eig_prim.argtypes = [pp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp]
         for f in ('oracle_hll_flux', 'oracle_rusanov_flux', 'oracle_hllc_flux', 'oracle_marquina_flux'):
             getattr(L, f).argtypes = [pp, _dp, _dp, ctypes.c_int, _dp]
         _lib = L

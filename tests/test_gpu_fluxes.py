@@ -161,7 +161,7 @@ def test_roe_entropy_fix(rec, fast):
     assert not np.array_equal(outs[0], outs[1])
 
 
-PLM_KINDS = ['plm prim', 'plm eig', 'plm cons with flux', 'plm athena', 'ppm']
+PLM_KINDS = ['plm prim', 'plm eig', 'plm cons with flux', 'plm athena', 'ppm', 'plm eig prim', 'plm eig prim ref']
 
 
 @pytest.mark.parametrize('real,fast', [('double', False), ('double', True), ('float', True)])
@@ -172,7 +172,8 @@ def test_plm_kinds_2d(plm, real, fast):
     edge fluxes; 'plm cons with flux' (plm.cl:95-188): cons slopes with the
     same half step; 'plm athena' (plm.cl:782-879): Athena's characteristic
     slope of the primitives with clamped edges; 'ppm' (plm.cl:882-944): the
-    kappa = 1/3 edges; all with the Roe flux"""
+    kappa = 1/3 edges; 'plm eig prim' / 'ref' (plm.cl:536-780): characteristic
+    slopes of the primitives; all with the Roe flux"""
     kw = dict(dim=2, gridSize=(150, 45), usePLM=plm, slopeLimiter='van Leer', integrator='Runge-Kutta 2, TVD',
               cfl=.3, boundary={'xmin': 'mirror', 'xmax': 'freeflow', 'ymin': 'periodic', 'ymax': 'periodic'})
     s, o = _pair(_quad(12), 8, real=real, fastMath=fast, **kw)

@@ -692,6 +692,122 @@ def test_plm_athena_against_eigendecomposition(vx):
             assert np.allclose(got(Rr), c3(lv), rtol=1e-11, atol=1e-13), (trial, side)
 
 
+def _plm_eig_prim(Um, U, Up, side=0, dt_dx=.3, ref=0):
+    L, Rr = np.zeros(6), np.zeros(6)
+    lib().oracle_plm_eig_prim(ctypes.byref(P), ref, _ptr(np.asarray(Um, float)), _ptr(np.asarray(U, float)),
+                              _ptr(np.asarray(Up, float)), side, dt_dx, _ptr(L), _ptr(Rr))
+    return L, Rr
+
+
+def _prim_jacobian_1d(w, gamma=1.4):
+    """textbook Jacobian of the 1D Euler equations in (rho, u, P)"""
+    rho, u, p = w
+    return np.array([[u, rho, 0], [0, u, 1 / rho], [0, gamma * p, u]])
+
+
+def _cons3(w, gamma=1.4):
+    return np.array([w[0], w[0] * w[1], w[2] / (gamma - 1) + .5 * w[0] * w[1] ** 2])
+
+
+def _full(c3, side):
+    U = np.zeros(6)
+    U[0], U[1 + side], U[4] = c3
+    return U
+
+
+def test_plm_eig_prim_constant_and_supersonic():
+    """'plm eig prim' and 'plm eig prim ref': constant data gives back U.
+    Primitives linear across the three cells (slope dW), all waves moving
+    right: the doubled minmod-of-three slope is 2 dW, so the left edge gets
+    W + dW - dt/dx A_W dW (A_W the textbook Jacobian in primitives) and the
+    right edge W; all moving left: the left edge W, the right edge
+    W - dW - dt/dx A_W dW. In the 'ref' form the fastest wave's reference
+    state cancels the slope term on the side it leaves: with all waves right
+    the edge states are written out below with waveMin = min(0, u - c),
+    waveMax = max(0, u + c). A slope whose density is below rhoMin keeps
+    only its density (the reference's dW/dU)"""
+    def low(s):
+        return s if s[0] >= 1e-7 else np.array([s[0], 0., 0.])
+    U = _cons(1.3, (.2, -.1, .05), .8)
+    for ref in (0, 1):
+        L, Rr = _plm_eig_prim(U, U, U, ref=ref)
+        assert np.allclose(L, U, rtol=1e-15) and np.allclose(Rr, U, rtol=1e-15)
+    dW = np.array([.05, .02, .03])
+    dt_dx = .13
+    for u in (3., -3.):   # |u| > c = sqrt(1.4 * .8 / 1.3) ~ .93
+        W0 = np.array([1.3, u, .8])
+        A = _prim_jacobian_1d(W0)
+        lam = np.linalg.eigvals(A)
+        for side in range(3):
+            cells = [_full(_cons3(W0 + k * dW), side) for k in (-1, 0, 1)]
+            L, Rr = _plm_eig_prim(*cells, side, dt_dx, 0)
+            got = lambda X: X[[0, 1 + side, 4]]   # noqa: E731
+            if u > 0:
+                wantL, wantR = W0 + low(dW - dt_dx * A @ dW), W0
+            else:
+                wantL, wantR = W0, W0 - low(dW + dt_dx * A @ dW)
+            assert np.allclose(got(L), _cons3(wantL), rtol=1e-12), (u, side)
+            assert np.allclose(got(Rr), _cons3(wantR), rtol=1e-12), (u, side)
+            # ref: dWM = 2 dW; WLRef = W + (1 - dt_dx waveMax) dW, WRRef = W - (1 + dt_dx waveMin) dW
+            L, Rr = _plm_eig_prim(*cells, side, dt_dx, 1)
+            wmin, wmax = min(0., lam.min()), max(0., lam.max())
+            I = np.eye(3)
+            if u > 0:   # aL = 2a dt_dx (waveMax - lambda), aR = 0
+                wantL = W0 + (1 - dt_dx * wmax) * dW + .5 * low(2 * dt_dx * (wmax * I - A) @ dW)
+                wantR = W0 - (1 + dt_dx * wmin) * dW
+            else:       # aL = 0, aR = 2a dt_dx (waveMin - lambda)
+                wantL = W0 + (1 - dt_dx * wmax) * dW
+                wantR = W0 - (1 + dt_dx * wmin) * dW + .5 * low(2 * dt_dx * (wmin * I - A) @ dW)
+            assert np.allclose(got(L), _cons3(wantL), rtol=1e-12), ('ref', u, side)
+            assert np.allclose(got(Rr), _cons3(wantR), rtol=1e-12), ('ref', u, side)
+
+
+@pytest.mark.parametrize('ref', [0, 1])
+@pytest.mark.parametrize('vx', [.35, -.5, 0.])
+def test_plm_eig_prim_against_eigendecomposition(ref, vx):
+    """'plm eig prim' / 'ref' in 1D-shaped data against numpy's
+    eigendecomposition of the textbook Jacobian in primitives (the
+    characteristic slopes, the doubled minmod-of-three, the per-wave edge
+    weights, and the slopes back; the slope steps do not depend on the
+    eigenvectors' scaling); a difference whose density is below rhoMin maps
+    to zero velocity and pressure, as the reference's dW/dU does"""
+    rng = np.random.default_rng(41)
+    dt_dx = .21
+    for trial in range(6):
+        W = [np.array([1 + .4 * rng.random(), vx + (.2 * rng.random() if vx else 0.), .7 + .3 * rng.random()])
+             for _ in range(3)]
+        A = _prim_jacobian_1d(W[1])
+        lam, V = np.linalg.eig(A)
+        lam, V = lam.real, V.real
+        if vx == 0:
+            lam = np.where(np.abs(lam) < 1e-12, 0., lam)
+        Vi = np.linalg.inv(V)
+        l, r, c = Vi @ (W[1] - W[0]), Vi @ (W[2] - W[1]), Vi @ (.5 * (W[2] - W[0]))
+        dM = np.where(l * r < 0, 0., np.where(c >= 0, 1., -1.) * 2 * np.minimum(np.minimum(abs(l), abs(r)), abs(c)))
+
+        def back(a):
+            s = V @ a
+            return s if s[0] >= 1e-7 else np.array([s[0], 0., 0.])
+        if not ref:
+            aL = np.where(lam < 0, 0., dM * .5 * (1 - lam * dt_dx))
+            aR = np.where(lam > 0, 0., dM * .5 * (1 + lam * dt_dx))
+            w2L, w2R = W[1] + back(aL), W[1] - back(aR)
+        else:
+            wmin, wmax = min(0., lam.min()), max(0., lam.max())
+            dWM = back(dM)
+            WLRef = W[1] + .5 * (1 - dt_dx * wmax) * dWM
+            WRRef = W[1] - .5 * (1 + dt_dx * wmin) * dWM
+            aL = np.where(lam < 0, 0., dM * dt_dx * (wmax - lam))
+            aR = np.where(lam > 0, 0., dM * dt_dx * (wmin - lam))
+            w2L, w2R = WLRef + .5 * back(aL), WRRef + .5 * back(aR)
+        for side in range(3):
+            cells = [_full(_cons3(w), side) for w in W]
+            L, Rr = _plm_eig_prim(*cells, side, dt_dx, ref)
+            got = lambda X: X[[0, 1 + side, 4]]   # noqa: E731
+            assert np.allclose(got(L), _cons3(w2L), rtol=1e-11, atol=1e-13), (trial, side)
+            assert np.allclose(got(Rr), _cons3(w2R), rtol=1e-11, atol=1e-13), (trial, side)
+
+
 def test_plm_cons_cases():
     c = np.array([1., 2, 3, 4, 5, 6])
     L, R = _plm(c, c, c)

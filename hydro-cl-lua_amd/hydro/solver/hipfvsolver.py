@@ -4,7 +4,7 @@ Mirrors `require 'hydro.solver.fvsolver'(args)` (hydro/solver/fvsolver.lua,
 gridsolver.lua, solverbase.lua): the same construction arguments (dim,
 gridSize, mins, maxs, eqn, flux, fluxLimiter, usePLM, slopeLimiter,
 integrator, cfl, fixedDT, boundary, initCond) and the methods the app calls —
-update, step(dt), calcDT, boundary, resetState, constrainU — plus
+update, step(dt), calcDT, boundary, resetState, constrainU, save — plus
 UBufObj:toCPU()/fromCPU() on the ghosted cons_t buffer, and t / dt.
 Errors raise like the reference's asserts.
 """
@@ -172,6 +172,18 @@ class HipFVSolver:
         out = (ctypes.c_double * 4)()
         _abi.check(_abi.lib().hydro_var_stats(self.h, int(field), out))
         return tuple(out[:])
+
+    def saveBuffer(self, buffer, basefn):
+        """GridSolver:saveBuffer (gridsolver.lua:1410-1455): the ghosted
+        buffer (z, y, x, channels) as `<basefn>.fits`, one plane per channel"""
+        from .. import fits
+        return fits.save_buffer(buffer.toCPU() if hasattr(buffer, 'toCPU') else buffer, basefn)
+
+    def save(self, prefix=None):
+        """GridSolver:save (gridsolver.lua:1465-1470) for the buffer this
+        solver holds, UBuf: writes `<prefix>_UBuf.fits` (`UBuf.fits` without
+        a prefix) and returns the file names"""
+        return [self.saveBuffer(self.UBufObj, (prefix + '_' if prefix else '') + 'UBuf')]
 
     def timeUpdates(self, n, perKernel=True):
         t = _abi.HydroTiming()

@@ -84,3 +84,25 @@ def test_stats_fp32():
     from hydro._abi import HydroError
     with pytest.raises(HydroError):
         s.calcDisplayVarRangeAndAvgAndStdDev(5)
+
+
+def test_save_fits_matches_download(tmp_path):
+    """GridSolver:save (gridsolver.lua:1465-1470) of the state: the FITS
+    file holds the downloaded ghosted buffer in the reference's plane layout,
+    2D and 3D (z interleaved along x), FP64 and FP32"""
+    from hydro import fits
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as ie
+    for kw in (dict(dim=2, gridSize=(40, 24)), dict(dim=3, gridSize=(20, 12, 8), real='float')):
+        s = HipFVSolver(integrator='Runge-Kutta 2, TVD', boundary='freeflow', **kw)
+        s.resetState(ie.to_ghosted(s.cfg, ie.quadrant(s.cfg, ie.random_quadrant_states(np.random.default_rng(4)))).astype(s.dtype))
+        s.update(3)
+        names = s.save(str(tmp_path / f'results{kw["dim"]}'))
+        assert names == [str(tmp_path / f'results{kw["dim"]}_UBuf.fits')]
+        planes, hdr = fits.read_image(names[0])
+        U = s.UBufObj.toCPU()
+        d, h, w, c = U.shape
+        assert (hdr['NAXIS1'], hdr['NAXIS2'], hdr['NAXIS3']) == (w * d, h, c)
+        assert planes.dtype == U.dtype
+        assert np.array_equal(planes, fits.buffer_planes(U))
+        assert np.array_equal(planes[0, :, :w], U[0, :, :, 0])

@@ -134,9 +134,44 @@ __global__ void __launch_bounds__(TOTALS_BLOCK) totals_partial_kernel(const R *U
 // values), the average and the population standard deviation over the
 // interior (the reference zeroes the ghosts for its sums and divides by the
 // interior count). Two passes, FP64 compensated sums.
+// Derived Euler display variables (euler.lua:240-260) of one cell's cons
+// (rho, m, ETotal) in double: W from primFromCons (euler.cl:140-159, P by
+// calc_P, euler.cl:63-83), then the variable's formula from euler.cl:1-94.
+struct StatsPar {
+	double gamma, rhoMin, PMin;
+};
+__device__ inline double euler_display(const StatsPar &p, int id, const double *U) {
+	const double rho = U[0];
+	const double inv = 1. / rho;
+	const double v[3] = {U[1] * inv, U[2] * inv, U[3] * inv};
+	const double mSq = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+	const double EKinC = rho < p.rhoMin ? 0. : (.5 * mSq / rho);
+	const double P = rho < p.rhoMin ? 0. : ((p.gamma - 1.) * (U[4] - EKinC));
+	const double vSq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+	auto Cs = [&]() { return P <= p.PMin ? 0. : rho < p.rhoMin ? (double)INFINITY : sqrt(p.gamma * P / rho); };
+	switch (id) {
+	case HYDRO_VAR_P: return P;
+	case HYDRO_VAR_E_INT_SPEC: return P / (p.gamma - 1.) / rho;
+	case HYDRO_VAR_E_KIN_SPEC: return .5 * vSq;
+	case HYDRO_VAR_E_TOTAL_SPEC: return U[4] / rho;
+	case HYDRO_VAR_E_INT: return P / (p.gamma - 1.);
+	case HYDRO_VAR_E_KIN: return rho * (.5 * vSq);
+	case HYDRO_VAR_S: return P / pow(rho, p.gamma);
+	case HYDRO_VAR_H: return P * (p.gamma / (p.gamma - 1.));
+	case HYDRO_VAR_H_SPEC: return P * (p.gamma / (p.gamma - 1.)) / rho;
+	case HYDRO_VAR_H_TOTAL: return P + U[4];
+	case HYDRO_VAR_H_TOTAL_SPEC: return (P + U[4]) / rho;
+	case HYDRO_VAR_CS: return Cs();
+	case HYDRO_VAR_MACH: return sqrt(vSq) / Cs();
+	case HYDRO_VAR_V_X: return v[0];
+	case HYDRO_VAR_V_Y: return v[1];
+	default: return v[2];
+	}
+}
+
 template <typename R, template <typename> class EQ>
 __global__ void __launch_bounds__(TOTALS_BLOCK) stats_partial_kernel(const GridDesc<R> g, int dim, int field, int pass,
-	const double *out, KSum *partial)
+	const double *out, KSum *partial, StatsPar sp)
 {
 	using E = EQ<R>;
 	__shared__ KSum sh[TOTALS_BLOCK];
@@ -164,14 +199,28 @@ __global__ void __launch_bounds__(TOTALS_BLOCK) stats_partial_kernel(const GridD
 		}
 		if (pass && !interior) continue;
 		const long src = ((long)m[2].idx * g.n[1] + m[1].idx) * g.step1 + m[0].idx;
-		R v = field < E::Q ? g.U[0][field * g.ps + src] : g.stat[(field - E::Q) * g.ps + src];
 		// axis by axis as the boundary passes run: a fixed face writes its
 		// value, a mirror face flips the reflected fields
-		for (int a = 0; a < dim; ++a) {
-			if (m[a].fixedFace >= 0) v = g.fixedv[m[a].fixedFace][field];
-			else if (m[a].mirror && E::reflects(field, a)) v = -v;
+		auto fieldAt = [&](int f) {
+			R v = f < E::Q ? g.U[0][f * g.ps + src] : g.stat[(f - E::Q) * g.ps + src];
+			for (int a = 0; a < dim; ++a) {
+				if (m[a].fixedFace >= 0) v = g.fixedv[m[a].fixedFace][f];
+				else if (m[a].mirror && E::reflects(f, a)) v = -v;
+			}
+			return (double)v;
+		};
+		double x;
+		if constexpr (E::Q == 5) {
+			if (field >= HYDRO_VAR_P) {
+				double U[5];
+				for (int f = 0; f < 5; ++f) U[f] = fieldAt(f);
+				x = euler_display(sp, field, U);
+			} else {
+				x = fieldAt(field);
+			}
+		} else {
+			x = fieldAt(field);
 		}
-		const double x = (double)v;
 		if (pass == 0) {
 			mn = fmin(mn, x);
 			mx = fmax(mx, x);

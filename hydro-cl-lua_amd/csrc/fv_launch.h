@@ -58,7 +58,7 @@ hipError_t launch_totals(const R *U, long ps, long ncell, int q, KSum *partial, 
 // on the device; partial: 2 nPartial KSums)
 template <typename R>
 hipError_t launch_stats(int eqn, int dim, const GridDesc<R> &g, int field, KSum *partial, int nPartial, double nInterior,
-	double *out, hipStream_t s);
+	double *out, const double *eqnPar /* gamma, rhoMin, PMin */, hipStream_t s);
 
 hipError_t launch_reset_slots(DevScalars *dev, int mask, hipStream_t s);
 hipError_t launch_fold_slot(DevScalars *dev, int slot, hipStream_t s);

@@ -238,17 +238,23 @@ extern "C" int hydro_set_time(hydro_solver *h, double t) {
 extern "C" int hydro_var_stats(hydro_solver *h, int field, double *out) {
 	if (!h || !out) return fail("NULL argument");
 	Grid &g = h->g;
-	if (field < 0 || field >= g.Q + g.NSTAT) return fail("field %d out of range 0..%d", field, g.Q + g.NSTAT - 1);
+	const bool derived = field >= HYDRO_VAR_P && field < HYDRO_VAR_END;
+	if (derived && g.cfg.eqn != HYDRO_EQN_EULER) return fail("the HYDRO_VAR_* display variables are the Euler equations'");
+	if (!derived && (field < 0 || field >= g.Q + g.NSTAT))
+		return fail("field %d out of range 0..%d or HYDRO_VAR_*", field, g.Q + g.NSTAT - 1);
+	const double eqnPar[3] = {g.cfg.gamma, g.cfg.rho_min, g.cfg.P_min};
 	double nInterior = (double)g.ncell;
 	hipError_t e;
 	if (g.elem == 8) {
 		GridDesc<double> d;
 		grid_desc<double>(g, d);
-		e = launch_stats<double>(g.cfg.eqn, g.dim, d, field, h->partial, h->nPartial, nInterior, h->totalsDev, h->stream);
+		e = launch_stats<double>(g.cfg.eqn, g.dim, d, field, h->partial, h->nPartial, nInterior, h->totalsDev, eqnPar,
+			h->stream);
 	} else {
 		GridDesc<float> d;
 		grid_desc<float>(g, d);
-		e = launch_stats<float>(g.cfg.eqn, g.dim, d, field, h->partial, h->nPartial, nInterior, h->totalsDev, h->stream);
+		e = launch_stats<float>(g.cfg.eqn, g.dim, d, field, h->partial, h->nPartial, nInterior, h->totalsDev, eqnPar,
+			h->stream);
 	}
 	if (e != hipSuccess) return fail("stats kernel: %s", hipGetErrorString(e));
 	HIPCHECK(hipMemcpyAsync(out, h->totalsDev, sizeof(double) * 4, hipMemcpyDeviceToHost, h->stream));

@@ -38,6 +38,12 @@ PLM_PIECEWISE_CONSTANT = 'piecewise constant'
 PRECISIONS = ['double', 'float']
 FACES = ['xmin', 'xmax', 'ymin', 'ymax', 'zmin', 'zmax']
 
+# derived display variables of the Euler equations (euler.lua:240-260), the
+# HYDRO_VAR_* ids of hydro_var_stats
+EULER_DISPLAY_VARS = {name: 64 + i for i, name in enumerate([
+    'P', 'eInt', 'eKin', 'eTotal', 'EInt', 'EKin', 'S', 'H', 'h', 'HTotal', 'hTotal',
+    'speed of sound', 'Mach number', 'v x', 'v y', 'v z'])}
+
 EQN_NUM_STATES = {'euler': 6, 'glm_maxwell': 12}
 EQN_NUM_INT_STATES = {'euler': 5, 'glm_maxwell': 8}
 

@@ -16,7 +16,7 @@ from .. import _abi
 from .. import eqn as eqn_plugins
 from .. import flux as flux_plugins
 from .. import int as int_plugins
-from .._cfg import make_cfg, cfg_ghost_shape, INTEGRATOR_STAGES
+from .._cfg import make_cfg, cfg_ghost_shape, INTEGRATOR_STAGES, EULER_DISPLAY_VARS
 from ..init import euler as init_euler
 
 
@@ -167,8 +167,14 @@ class HipFVSolver:
     def calcDisplayVarRangeAndAvgAndStdDev(self, field):
         """(min, max, avg, stddev) of the stored cons_t field `field` (index),
         as SolverBase:calcDisplayVarRangeAndAvgAndStdDev (solverbase.lua:
-        2871-2994) gives them for the display var 'U <field>': min/max over
-        the ghosted grid, avg/stddev over the interior; computed on the GPU"""
+        2871-2994) gives them for the display var 'U <field>', or of a derived
+        Euler display variable by its name ('P', 'eInt', ..., 'Mach number',
+        'v x'; euler.lua:240-260): min/max over the ghosted grid, avg/stddev
+        over the interior; computed on the GPU"""
+        if isinstance(field, str):
+            if field not in EULER_DISPLAY_VARS:
+                raise ValueError(f'unknown display var {field!r}; expected one of {list(EULER_DISPLAY_VARS)}')
+            field = EULER_DISPLAY_VARS[field]
         out = (ctypes.c_double * 4)()
         _abi.check(_abi.lib().hydro_var_stats(self.h, int(field), out))
         return tuple(out[:])

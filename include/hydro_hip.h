@@ -209,8 +209,31 @@ int hydro_set_time(hydro_solver *h, double t);
  * out has numIntStates entries. */
 int hydro_conserved_totals(hydro_solver *h, double *out);
 
+/* Derived display variables of the Euler equations (euler.lua:240-260), for
+ * hydro_var_stats: W = primFromCons(U) per cell, values in double. */
+enum {
+	HYDRO_VAR_P = 64,          /* 'P' */
+	HYDRO_VAR_E_INT_SPEC,      /* 'eInt' = P / (gamma - 1) / rho */
+	HYDRO_VAR_E_KIN_SPEC,      /* 'eKin' = |v|^2 / 2 */
+	HYDRO_VAR_E_TOTAL_SPEC,    /* 'eTotal' = ETotal / rho */
+	HYDRO_VAR_E_INT,           /* 'EInt' = P / (gamma - 1) */
+	HYDRO_VAR_E_KIN,           /* 'EKin' = rho |v|^2 / 2 */
+	HYDRO_VAR_S,               /* 'S' = P / rho^gamma */
+	HYDRO_VAR_H,               /* 'H' = P gamma / (gamma - 1) */
+	HYDRO_VAR_H_SPEC,          /* 'h' = H / rho */
+	HYDRO_VAR_H_TOTAL,         /* 'HTotal' = P + ETotal */
+	HYDRO_VAR_H_TOTAL_SPEC,    /* 'hTotal' = HTotal / rho */
+	HYDRO_VAR_CS,              /* 'speed of sound' (calc_Cs, euler.cl:87-94) */
+	HYDRO_VAR_MACH,            /* 'Mach number' = |v| / Cs */
+	HYDRO_VAR_V_X,             /* 'v x', 'v y', 'v z' */
+	HYDRO_VAR_V_Y,
+	HYDRO_VAR_V_Z,
+	HYDRO_VAR_END
+};
+
 /* The display statistics of one stored cons_t field (0 .. numIntStates +
- * static fields - 1), calcDisplayVarRangeAndAvgAndStdDev
+ * static fields - 1), or of a derived Euler variable (HYDRO_VAR_*),
+ * calcDisplayVarRangeAndAvgAndStdDev
  * (hydro/solver/solverbase.lua:2871-2994) of the display var 'U <field>':
  * out[0], out[1] = min and max over every cell of the ghosted grid, ghosts as
  * the boundary fills them; out[2] = average and out[3] = population standard

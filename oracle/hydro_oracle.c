@@ -1266,17 +1266,49 @@ static void boundary_buf(oracle_solver *s, real *buf) {
 
 void oracle_boundary(oracle_solver *s) { boundary_buf(s, s->U); }
 
+double oracle_euler_display(const oracle_params *p, int id, const double *U) {
+	/* euler.lua:240-260 with W = primFromCons(U) (euler.cl:140-159) and the
+	 * formulas of euler.cl:1-94 */
+	real W[6];
+	oracle_euler_prim_from_cons(p, U, W);
+	real g = p->gamma;
+	real vSq = dot3(W + 1, W + 1);
+	switch (id) {
+	case HYDRO_VAR_P: return W[4];
+	case HYDRO_VAR_E_INT_SPEC: return W[4] / (g - 1.) / W[0];
+	case HYDRO_VAR_E_KIN_SPEC: return .5 * vSq;
+	case HYDRO_VAR_E_TOTAL_SPEC: return U[4] / W[0];
+	case HYDRO_VAR_E_INT: return W[4] / (g - 1.);
+	case HYDRO_VAR_E_KIN: return W[0] * (.5 * vSq);
+	case HYDRO_VAR_S: return W[4] / pow(W[0], g);
+	case HYDRO_VAR_H: return W[4] * (g / (g - 1.));
+	case HYDRO_VAR_H_SPEC: return W[4] * (g / (g - 1.)) / W[0];
+	case HYDRO_VAR_H_TOTAL: return W[4] + U[4];
+	case HYDRO_VAR_H_TOTAL_SPEC: return (W[4] + U[4]) / W[0];
+	case HYDRO_VAR_CS: return euler_calc_Cs(p, W);
+	case HYDRO_VAR_MACH: return sqrt(vSq) / euler_calc_Cs(p, W);
+	case HYDRO_VAR_V_X: return W[1];
+	case HYDRO_VAR_V_Y: return W[2];
+	case HYDRO_VAR_V_Z: return W[3];
+	default: return NAN;
+	}
+}
+
+static real var_value(const oracle_solver *s, long c, int field) {
+	if (field >= HYDRO_VAR_P) return oracle_euler_display(&s->p, field, s->U + c * s->S);
+	return s->U[c * s->S + field];
+}
+
 void oracle_var_stats(const oracle_solver *s, int field, double *out) {
 	/* solverbase.lua:2871-2994: reduceMin/reduceMax over the whole buffer;
 	 * reduceSum over the buffer with zeroed ghosts, divided by
 	 * volumeWithoutBorder; subtractAndSquare on the interior (solverbase.lua:
 	 * 1269-1278) for the standard deviation */
-	int S = s->S;
 	long n = 1, ni = 1;
 	for (int a = 0; a < 3; ++a) { n *= s->gs[a]; if (a < s->dim) ni *= s->gs[a] - 2 * NG; }
 	real mn = INFINITY, mx = -INFINITY, sum = 0;
 	for (long c = 0; c < n; ++c) {
-		real v = s->U[c * S + field];
+		real v = var_value(s, c, field);
 		mn = fmin(mn, v);
 		mx = fmax(mx, v);
 	}
@@ -1288,13 +1320,13 @@ void oracle_var_stats(const oracle_solver *s, int field, double *out) {
 	/* serial loops: one fixed summation order */
 	for (int k = lo[2]; k < hi[2]; ++k)
 		for (int j = lo[1]; j < hi[1]; ++j)
-			for (int i = lo[0]; i < hi[0]; ++i) sum += s->U[(i + s->step[1] * j + s->step[2] * k) * S + field];
+			for (int i = lo[0]; i < hi[0]; ++i) sum += var_value(s, i + s->step[1] * j + s->step[2] * k, field);
 	real avg = sum / (real)ni;
 	real sq = 0;
 	for (int k = lo[2]; k < hi[2]; ++k)
 		for (int j = lo[1]; j < hi[1]; ++j)
 			for (int i = lo[0]; i < hi[0]; ++i) {
-				real d = s->U[(i + s->step[1] * j + s->step[2] * k) * S + field] - avg;
+				real d = var_value(s, i + s->step[1] * j + s->step[2] * k, field) - avg;
 				sq += d * d;
 			}
 	out[0] = mn; out[1] = mx; out[2] = avg; out[3] = sqrt(sq / (real)ni);

@@ -41,6 +41,8 @@ void oracle_boundary(oracle_solver *s);
 /* calcDisplayVarRangeAndAvgAndStdDev (solverbase.lua:2871-2994) of cons_t
  * field `field` of the ghosted state: out = {min, max over all cells, average,
  * population standard deviation over the interior} */
+/* a derived Euler display variable (HYDRO_VAR_*, euler.lua:240-260) of one cons_t */
+double oracle_euler_display(const oracle_params *p, int id, const double *U);
 void oracle_var_stats(const oracle_solver *s, int field, double *out);
 void oracle_constrain_U(oracle_solver *s);      /* constrainU kernel + boundary */
 double oracle_calc_dt(oracle_solver *s);        /* cfl * min (or fixed dt) */

@@ -77,6 +77,8 @@ def lib():
                                       _dp, _dp, ctypes.POINTER(_dp), ctypes.c_int, _dp]
         L.oracle_plm_cons.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.oracle_var_stats.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.oracle_euler_display.restype = ctypes.c_double
+        L.oracle_euler_display.argtypes = [pp, ctypes.c_int, _dp]
         L.oracle_plm_prim.argtypes = [pp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.oracle_plm_eig.argtypes = [pp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp]
         L.oracle_plm_cons_flux.argtypes = L.oracle_plm_eig.argtypes

@@ -106,3 +106,28 @@ def test_save_fits_matches_download(tmp_path):
         assert planes.dtype == U.dtype
         assert np.array_equal(planes, fits.buffer_planes(U))
         assert np.array_equal(planes[0, :, :w], U[0, :, :, 0])
+
+
+@pytest.mark.parametrize('real', ['double', 'float'])
+def test_stats_display_vars(real):
+    """the derived Euler display variables (euler.lua:240-260) on the GPU
+    against the oracle: FP64 within 1e-13 (pow of 'S' may differ in the last
+    place), the FP32 state within 1e-5 of the FP64 oracle"""
+    from hydro._cfg import EULER_DISPLAY_VARS
+    kw = dict(dim=2, gridSize=(70, 33), integrator='Runge-Kutta 2, TVD', cfl=.3,
+              boundary={'xmin': 'fixed', 'xmax': 'mirror', 'ymin': 'freeflow', 'ymax': 'periodic'},
+              boundaryFixed={'xmin': [2.5, .6, .1, 0, 6., 0]}, usePLM='plm cons', slopeLimiter='minmod')
+    s, o = _pair(_quad(19), 5, real=real, **kw)
+    s.update(5)
+    o.update(5)
+    tol = 1e-13 if real == 'double' else 1e-5
+    for name, vid in EULER_DISPLAY_VARS.items():
+        got = np.array(s.calcDisplayVarRangeAndAvgAndStdDev(name))
+        ref = _stats_oracle(o, vid)
+        print(name, got, ref)
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.allclose(got, ref, rtol=tol, atol=tol * scale), (name, got, ref)
+    import pytest as _pt
+    from hydro import _abi
+    with _pt.raises(_abi.HydroError):
+        s.calcDisplayVarRangeAndAvgAndStdDev(200)

@@ -1146,6 +1146,56 @@ def test_glm_source_uniform_fields(plm):
     assert np.allclose(d[..., :8], want, rtol=1e-14, atol=1e-14)
 
 
+def _display_textbook(U, name, gamma=1.4):
+    rho, m, E = U[0], np.asarray(U[1:4]), U[4]
+    v = m / rho
+    P = (gamma - 1) * (E - .5 * rho * v @ v)
+    cs = np.sqrt(gamma * P / rho)
+    return {'P': P, 'eInt': P / (gamma - 1) / rho, 'eKin': .5 * v @ v, 'eTotal': E / rho, 'EInt': P / (gamma - 1),
+            'EKin': .5 * rho * v @ v, 'S': P / rho ** gamma, 'H': gamma * P / (gamma - 1),
+            'h': gamma * P / (gamma - 1) / rho, 'HTotal': P + E, 'hTotal': (P + E) / rho, 'speed of sound': cs,
+            'Mach number': np.sqrt(v @ v) / cs, 'v x': v[0], 'v y': v[1], 'v z': v[2]}[name]
+
+
+def test_euler_display_vars():
+    """the derived display variables against textbook formulas of the
+    primitives; below PMin the speed of sound is 0 (Mach number inf), below
+    rhoMin P is 0 and Cs is infinite (euler.cl:77-94)"""
+    from hydro._cfg import EULER_DISPLAY_VARS
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        U = _cons(rng.uniform(.1, 3), rng.uniform(-1, 1, 3), rng.uniform(.1, 3))
+        for name, vid in EULER_DISPLAY_VARS.items():
+            got = lib().oracle_euler_display(ctypes.byref(P), vid, _ptr(U))
+            assert np.isclose(got, _display_textbook(U, name), rtol=1e-13, atol=1e-15), name
+    U = _cons(1., (.5, 0, 0), 0.)
+    assert lib().oracle_euler_display(ctypes.byref(P), EULER_DISPLAY_VARS['speed of sound'], _ptr(U)) == 0
+    assert np.isinf(lib().oracle_euler_display(ctypes.byref(P), EULER_DISPLAY_VARS['Mach number'], _ptr(U)))
+    U = np.array([1e-9, 0, 0, 0, 1., 0])
+    assert lib().oracle_euler_display(ctypes.byref(P), EULER_DISPLAY_VARS['P'], _ptr(U)) == 0
+
+
+def test_var_stats_of_display_vars():
+    """the statistics of a derived variable: min/max over the ghosted grid,
+    mean and population std over the interior, of the textbook values"""
+    from hydro._cfg import EULER_DISPLAY_VARS
+    cfg = make_cfg(dim=2, size=(20, 13), boundary='mirror', integrator='forward Euler', cfl=.3)
+    o = pyoracle.Oracle(cfg)
+    rng = np.random.default_rng(13)
+    o.set_U(init_euler.to_ghosted(cfg, init_euler.quadrant(cfg, init_euler.random_quadrant_states(rng))))
+    o.boundary()
+    o.constrain_U()
+    o.update(2)
+    out = np.zeros(4)
+    for name in ('P', 'hTotal', 'Mach number', 'v y', 'S'):
+        lib().oracle_var_stats(o.h, EULER_DISPLAY_VARS[name], _ptr(out))
+        allv = np.array([_display_textbook(u, name) for u in o.U.reshape(-1, 6)])
+        intv = np.array([_display_textbook(u, name) for u in o.interior().reshape(-1, 6)])
+        assert np.allclose(out[:2], [allv.min(), allv.max()], rtol=1e-12), name
+        assert np.isclose(out[2], intv.mean(), rtol=1e-12), name
+        assert np.isclose(out[3], intv.std(), rtol=1e-9), name
+
+
 def test_var_stats_are_min_max_mean_std():
     """calcDisplayVarRangeAndAvgAndStdDev: min/max over the ghosted grid
     (mirror ghosts flip the normal momenta, so they can hold the extremes),

@@ -127,7 +127,6 @@ def test_stats_display_vars(real):
         print(name, got, ref)
         scale = max(np.abs(ref).max(), 1e-300)
         assert np.allclose(got, ref, rtol=tol, atol=tol * scale), (name, got, ref)
-    import pytest as _pt
     from hydro import _abi
-    with _pt.raises(_abi.HydroError):
+    with pytest.raises(_abi.HydroError):
         s.calcDisplayVarRangeAndAvgAndStdDev(200)

@@ -14,7 +14,9 @@
 
 namespace hydro {
 
-template <bool FAST>
+// PLMK: HYDRO_PLM_CONS, or HYDRO_PLM_PPM (plm.cl:882-944), which like
+// 'plm cons' takes the integrated fields' differences only.
+template <bool FAST, int PLMK = HYDRO_PLM_CONS>
 struct GLMT {
 template <typename R>
 struct Eqn {
@@ -25,7 +27,7 @@ struct Eqn {
 	static constexpr bool PLM_EIG = false;
 	static constexpr bool PLM_CONS_FLUX = false;
 	static constexpr bool PLM_ATHENA = false;
-	static constexpr bool PLM_PPM = false;
+	static constexpr bool PLM_PPM = PLMK == HYDRO_PLM_PPM;
 	static constexpr bool PLM_EIG_PRIM = false;
 	static constexpr bool PLM_EIG_PRIM_REF = false;
 	static constexpr int Q = 8;

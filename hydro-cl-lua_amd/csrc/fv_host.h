@@ -170,7 +170,8 @@ static int validate(const hydro_cfg *c) {
 	if (c->slope_limiter < 0 || c->slope_limiter >= HYDRO_NUM_LIMITERS) return fail("slope_limiter %d out of range", c->slope_limiter);
 	if (c->plm < HYDRO_PLM_NONE || c->plm > HYDRO_PLM_EIG_PRIM_REF)
 		return fail("plm %d not supported", c->plm);
-	if (c->plm > HYDRO_PLM_CONS && (c->eqn != HYDRO_EQN_EULER || c->flux != HYDRO_FLUX_ROE || c->roe_entropy_fix))
+	const bool glmPpm = c->plm == HYDRO_PLM_PPM && c->eqn == HYDRO_EQN_GLM_MAXWELL;
+	if (c->plm > HYDRO_PLM_CONS && !glmPpm && (c->eqn != HYDRO_EQN_EULER || c->flux != HYDRO_FLUX_ROE || c->roe_entropy_fix))
 		return fail("%s is built for the Euler equations with the Roe flux (no entropy fix)",
 			c->plm == HYDRO_PLM_PRIM ? "plm prim" : c->plm == HYDRO_PLM_EIG ? "plm eig"
 			: c->plm == HYDRO_PLM_CONS_FLUX ? "plm cons with flux" : c->plm == HYDRO_PLM_ATHENA ? "plm athena"

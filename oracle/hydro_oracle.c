@@ -1126,8 +1126,8 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 	if (cfg->flux < HYDRO_FLUX_ROE || cfg->flux > HYDRO_FLUX_MARQUINA) {
 		set_err("bad flux"); return NULL;
 	}
-	if (cfg->plm > HYDRO_PLM_CONS && cfg->eqn != HYDRO_EQN_EULER) {
-		set_err("the reconstructions other than plm cons are built for the Euler equations"); return NULL;
+	if (cfg->plm > HYDRO_PLM_CONS && cfg->plm != HYDRO_PLM_PPM && cfg->eqn != HYDRO_EQN_EULER) {
+		set_err("the reconstructions other than plm cons and ppm are built for the Euler equations"); return NULL;
 	}
 	if (cfg->flux != HYDRO_FLUX_ROE && cfg->eqn != HYDRO_EQN_EULER) {
 		set_err("HLL, Rusanov, HLLC and Marquina take the Euler equations"); return NULL;

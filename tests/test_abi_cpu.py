@@ -55,7 +55,7 @@ def test_create_rejects_plm_kinds_outside_euler_roe():
     L = _abi.lib()
     for kw in (dict(eqn='glm_maxwell', plm='plm eig'), dict(flux='hll', plm='plm eig'),
                dict(eqn='glm_maxwell', plm='plm cons with flux'), dict(flux='marquina', plm='plm cons with flux'),
-               dict(eqn='glm_maxwell', plm='ppm'), dict(flux='rusanov', plm='plm athena'),
+               dict(flux='hll', plm='ppm'), dict(flux='rusanov', plm='plm athena'),
                dict(eqn='glm_maxwell', plm='plm eig prim'), dict(flux='euler-hllc', plm='plm eig prim ref'),
                dict(plm='plm eig', use_entropy_flux_fix=True), dict(eqn='glm_maxwell', plm='plm prim')):
         c = make_cfg(dim=2, size=(8, 8), **kw)

@@ -211,7 +211,7 @@ def _glm(seed, sigma=0.):
 
 
 @pytest.mark.parametrize('n,plm,bc,sigma', [((64, 48), 'plm cons', 'periodic', 0.), ((50, 40), None, 'freeflow', .5),
-                                             ((40, 33), 'plm cons', 'mirror', 1.)])
+                                             ((40, 33), 'plm cons', 'mirror', 1.), ((44, 37), 'ppm', 'mirror', .5)])
 def test_glm_maxwell_2d(n, plm, bc, sigma):
     kw = dict(dim=2, gridSize=n, eqn='glm_maxwell', usePLM=plm, slopeLimiter='minmod',
               integrator='Runge-Kutta 2, TVD', cfl=.4, boundary=bc)
@@ -224,6 +224,21 @@ def test_glm_maxwell_2d(n, plm, bc, sigma):
     print('glm', n, err)
     assert np.all(err <= TOL64), err
     assert np.array_equal(got[..., 8:], ref[..., 8:])   # static fields carried unchanged
+
+
+def test_glm_maxwell_1d_ppm():
+    """'ppm' (plm.cl:882-944) with the GLM-Maxwell equations: the kappa = 1/3
+    edges of the integrated fields, static fields copied"""
+    kw = dict(dim=1, gridSize=300, eqn='glm_maxwell', usePLM='ppm', slopeLimiter='van Leer',
+              integrator='Runge-Kutta 3, TVD', cfl=.4, boundary='freeflow')
+    s, o = _pair(_glm(12), 30, **kw)
+    s.update(30)
+    o.update(30)
+    got = _interior(1, s.UBufObj.toCPU())
+    ref = _interior(1, o.U)
+    err = rel_l1(got[..., :8], ref[..., :8])
+    print('glm ppm 1d', err)
+    assert np.all(err <= TOL64), err
 
 
 def test_glm_maxwell_1d_flux_limiter():

@@ -226,3 +226,18 @@ def test_plm_kinds_1d_3d_batch(plm):
         sp.resetState(U0s[i])
         sp.update(5)
         assert np.array_equal(b.download(i), sp.UBufObj.toCPU()), i
+
+
+def test_piecewise_constant_equals_no_reconstruction():
+    """'piecewise constant' (plm.cl:10-24) gives the edge states U itself:
+    the same bits as no reconstruction"""
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as ie
+    outs = []
+    for plm in ('piecewise constant', None):
+        s = HipFVSolver(dim=2, gridSize=(64, 40), usePLM=plm, integrator='Runge-Kutta 2, TVD', cfl=.4,
+                        boundary='periodic')
+        s.resetState(ie.to_ghosted(s.cfg, ie.quadrant(s.cfg, ie.random_quadrant_states(np.random.default_rng(6)))))
+        s.update(10)
+        outs.append(s.UBufObj.toCPU())
+    assert np.array_equal(outs[0], outs[1])

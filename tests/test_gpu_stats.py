@@ -130,3 +130,14 @@ def test_stats_display_vars(real):
     from hydro import _abi
     with pytest.raises(_abi.HydroError):
         s.calcDisplayVarRangeAndAvgAndStdDev(200)
+
+
+def test_display_vars_refused_for_glm():
+    """the HYDRO_VAR_* ids are the Euler equations' display variables"""
+    from hydro import _abi
+    from hydro.solver.hipfvsolver import HipFVSolver
+    s = HipFVSolver(dim=1, gridSize=32, eqn='glm_maxwell')
+    with pytest.raises(_abi.HydroError):
+        s.calcDisplayVarRangeAndAvgAndStdDev('P')
+    with pytest.raises(ValueError):
+        s.calcDisplayVarRangeAndAvgAndStdDev('no such var')

@@ -519,7 +519,14 @@ extern "C" int hydro_batch_get_time(hydro_batch *b, int i, double *t, double *dt
 
 extern "C" int hydro_batch_set_active(hydro_batch *b, int i, int active) {
 	if (batch_index(b, i)) return -1;
-	b->active[i] = active ? 1 : 0;
+	const int on = active ? 1 : 0;
+	// A grid that sat out kept none of the dt slots of the rotation current
+	// (inactive grids neither reset nor reduce them), so a grid switched back
+	// on recomputes dt from its state. batch_prepare then resets every grid's
+	// slots and takes each active grid's dt from its state: the same minimum
+	// its last stage reduced, since min is exact in any order.
+	if (on && !b->active[i]) b->dtValid = 0;
+	b->active[i] = on;
 	b->descsDirty = 1;
 	return 0;
 }

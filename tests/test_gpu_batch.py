@@ -126,3 +126,67 @@ def test_batch_autotune_keeps_results():
     for o in outs[1:]:
         for a, c in zip(outs[0], o):
             assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize('graph', [False, True])
+def test_batch_reactivated_grid_matches_oracle(graph):
+    """grids switched off for one, two and three steps (every offset of the
+    three-slot dt rotation) and back on continue from their own state with
+    their own dt: each grid's result equals the oracle's after the steps it
+    took part in"""
+    import pyoracle
+    from hydro.solver.batch import HipBatch
+    from hydro._cfg import make_cfg
+    kws = [_cfg(n) for n in SIZES[:4]]
+    b = HipBatch(kws)
+    U0s = [_init(make_cfg(**kw), 50 + i) for i, kw in enumerate(kws)]
+    for i, U0 in enumerate(U0s):
+        b.upload(i, U0)
+    b.constrainU()
+    taken = [0] * len(kws)
+    plan = [(2, ()), (1, (1, 2, 3)), (1, (2, 3)), (1, (3,)), (4, ()), (3, (0,)), (5, ())]
+    for steps, off in plan:
+        for i in range(len(kws)):
+            b.setActive(i, i not in off)
+        b.update(steps, graph=graph)
+        for i in range(len(kws)):
+            if i not in off:
+                taken[i] += steps
+    for i, kw in enumerate(kws):
+        o = pyoracle.Oracle(make_cfg(**kw))
+        o.set_U(U0s[i])
+        o.boundary()
+        o.constrain_U()
+        o.update(taken[i])
+        got = o.interior(b.download(i))[..., :5]
+        err = rel_l1(got, o.interior()[..., :5])
+        assert np.all(err <= 1e-12), (i, taken[i], err)
+        assert b.time(i)[0] == pytest.approx(o.t, rel=1e-14)
+
+
+@pytest.mark.parametrize('real,fast,tol', [('double', True, 1e-12), ('float', True, 1e-5)])
+def test_batch_glm_ppm_vs_oracle(real, fast, tol):
+    """GLM-Maxwell with 'ppm' in the batched kernels"""
+    import pyoracle
+    from hydro.solver.batch import HipBatch
+    from hydro._cfg import make_cfg
+    from hydro.init import glm_maxwell
+    from hydro.init import euler as ie
+    kws = [dict(dim=2, size=n, mins=(-1, -1), maxs=(1, 1), eqn='glm_maxwell', plm='ppm', slope_limiter='minmod',
+                integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic', precision=real, fast_math=fast)
+           for n in [(48, 48), (70, 33), (33, 80)]]
+    b = HipBatch(kws)
+    U0s = []
+    for i, kw in enumerate(kws):
+        cfg = make_cfg(**kw)
+        U0 = ie.to_ghosted(cfg, glm_maxwell.gaussian_pulses(cfg, seed=60 + i, conductivity=.5)).astype(b.dtype)
+        U0s.append(U0)
+        b.upload(i, U0)
+    b.update(20)
+    for i, kw in enumerate(kws):
+        o = pyoracle.Oracle(make_cfg(**dict(kw, precision='double', fast_math=False)))
+        o.set_U(U0s[i].astype(np.float64))
+        o.boundary()
+        o.update(20)
+        err = rel_l1(o.interior(b.download(i))[..., :8].astype(np.float64), o.interior()[..., :8])
+        assert np.all(err <= tol), (i, err)

@@ -1,0 +1,138 @@
+"""GPU parity on states that drive the reference's edge-case branches: gas
+expanding into vacuum (one- and two-sided vacuum Roe states, identity
+transforms below rhoMin, constrainU's density and pressure clamps), gas
+leaving a vacuum gap between two outgoing streams, and a cold gas (pressure
+below PMin, h < 1e-5 in the Roe average: Cs = 0), each in EXACT FP64, FAST
+FP64 and FP32, 1D, 2D along x and y, and 3D. The fast kernels handle these
+branches with wave-uniform tests (fv_euler.h), so these are their cases.
+
+Each case checks that the oracle's run entered the branch it is named for
+(vacuum cells, clamped densities, clamped pressures), and the HIP result
+against the oracle: FP64 <= 1e-12 relative L1 per conserved variable, FP32
+(state rounded to float on both sides at the start) <= 1e-5.
+
+The Roe flux with a reconstruction and the Roe flux on a cold gas are left
+out: the reference's arithmetic gives non-finite values there (the edge
+states of a strong rarefaction have negative pressure, and Cs = 0 above
+rhoMin divides by 0 in eigen_leftTransform, euler.cl:434-489), in the oracle
+as in the reference.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l1
+from gpu_util import interior, pair
+
+pytestmark = pytest.mark.gpu
+
+RHO_MIN = P_MIN = 1e-7
+
+
+def _expansion(axis):
+    """the lower half along `axis` at rest with rho = P = 1, the upper half
+    vacuum (rho = P = 1e-9, clamped to rhoMin / PMin)"""
+    def f(cfg):
+        from hydro.init import euler as ie
+        x, y, z = ie.mesh(cfg)
+        shape = ie.interior_shape(cfg)
+        vac = np.broadcast_to([x, y, z][axis] > 0, shape)
+        rho = np.where(vac, 1e-9, 1.)
+        zero = np.zeros(shape)
+        return ie.cons_from_prim(rho, (zero, zero, zero), np.where(vac, 1e-9, 1.), 1.4)
+    return f
+
+
+def _two_streams(axis):
+    """a vacuum gap |c| < .3 between streams moving apart at .5"""
+    def f(cfg):
+        from hydro.init import euler as ie
+        x, y, z = ie.mesh(cfg)
+        shape = ie.interior_shape(cfg)
+        c = np.broadcast_to([x, y, z][axis], shape)
+        gap = np.abs(c) < .3
+        v = [np.zeros(shape)] * 3
+        v[axis] = np.where(c < 0, -.5, .5) * ~gap
+        return ie.cons_from_prim(np.where(gap, 1e-9, 1.), v, np.where(gap, 1e-9, 1.), 1.4)
+    return f
+
+
+def _cold(cfg):
+    """density and velocity waves at P = 1e-9 (below PMin)"""
+    from hydro.init import euler as ie
+    x, y, z = ie.mesh(cfg)
+    shape = ie.interior_shape(cfg)
+    cy = np.cos(np.pi * y) if cfg.dim > 1 else 1.
+    rho = np.broadcast_to(1 + .5 * np.sin(np.pi * x) * cy, shape)
+    vx = np.broadcast_to(.5 * np.sin(2 * np.pi * x), shape)
+    vy = np.broadcast_to(.3 * np.cos(2 * np.pi * y), shape) if cfg.dim > 1 else 0 * vx
+    return ie.cons_from_prim(rho, (vx, vy, 0 * vx), 1e-9 + 0 * vx, 1.4)
+
+
+RK2 = dict(integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='freeflow')
+PLM = dict(usePLM='plm cons', slopeLimiter='minmod')
+
+# name, init, solver arguments, steps, what the oracle's run must show
+CASES = [
+    ('expansion-1d-roe', _expansion(0), dict(dim=1, gridSize=300, **RK2), 30, 'vacuum'),
+    ('expansion-1d-roe-superbee', _expansion(0), dict(dim=1, gridSize=300, fluxLimiter='superbee', **RK2), 30,
+     'vacuum'),
+    ('expansion-2d-x-roe', _expansion(0), dict(dim=2, gridSize=(150, 20), **RK2), 25, 'vacuum'),
+    ('expansion-2d-y-roe-superbee', _expansion(1), dict(dim=2, gridSize=(70, 90), fluxLimiter='superbee', **RK2),
+     25, 'vacuum'),
+    ('streams-2d-y-roe', _two_streams(1), dict(dim=2, gridSize=(70, 120), **RK2), 12, 'vacuum'),
+    ('expansion-3d-z-roe', _expansion(2), dict(dim=3, gridSize=(40, 12, 90), **RK2), 12, 'vacuum'),
+    ('expansion-3d-y-roe-vanleer', _expansion(1), dict(dim=3, gridSize=(70, 90, 6), fluxLimiter='van Leer', **RK2),
+     12, 'vacuum'),
+    ('expansion-2d-x-roe-plm-prim', _expansion(0), dict(dim=2, gridSize=(150, 20), usePLM='plm prim',
+                                                        slopeLimiter='minmod', **RK2), 25, 'vacuum'),
+    ('expansion-2d-x-hll-plm', _expansion(0), dict(dim=2, gridSize=(150, 20), flux='hll', **PLM, **RK2), 25,
+     'vacuum'),
+    ('expansion-2d-y-hllc-plm', _expansion(1), dict(dim=2, gridSize=(70, 90), flux='euler-hllc', **PLM, **RK2), 25,
+     'vacuum'),
+    ('expansion-2d-y-rusanov-plm', _expansion(1), dict(dim=2, gridSize=(70, 90), flux='rusanov', **PLM, **RK2), 25,
+     'vacuum'),
+    ('cold-2d-hll-plm', _cold, dict(dim=2, gridSize=(100, 80), flux='hll', **PLM, **dict(RK2, boundary='periodic')),
+     40, 'cold'),
+    ('cold-1d-hll-plm', _cold, dict(dim=1, gridSize=300, flux='hll', **PLM, **dict(RK2, boundary='periodic')), 40,
+     'cold'),
+]
+
+MODES = {'exact': dict(real='double', fastMath=False), 'fast': dict(real='double', fastMath=True),
+         'fp32': dict(real='float', fastMath=True)}
+
+
+def _pressure(U):
+    return .4 * (U[..., 4] - .5 * (U[..., 1] ** 2 + U[..., 2] ** 2 + U[..., 3] ** 2) / U[..., 0])
+
+
+@pytest.mark.parametrize('mode', list(MODES))
+@pytest.mark.parametrize('name,init,kw,steps,branch', CASES, ids=[c[0] for c in CASES])
+def test_edge_branches(name, init, kw, steps, branch, mode):
+    dim = kw['dim']
+    s, o = pair(init, **kw, **MODES[mode])
+    seen_vac = seen_rho_clamp = seen_p_clamp = 0
+    chunk = max(1, steps // 4)
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        # the oracle's state along the way enters the branches
+        I = interior(dim, o.U)
+        seen_vac += int((I[..., 0] < 1e-5).sum())
+        o.update(n)
+        I = interior(dim, o.U)
+        seen_rho_clamp += int((I[..., 0] == RHO_MIN).sum())
+        seen_p_clamp += int((np.abs(_pressure(I) - P_MIN) < 1e-3 * P_MIN).sum())
+        done += n
+    s.update(steps)
+    if branch == 'vacuum':
+        assert seen_vac > 0 and seen_rho_clamp > 0 and seen_p_clamp > 0, (seen_vac, seen_rho_clamp, seen_p_clamp)
+    else:
+        assert seen_p_clamp > 0, seen_p_clamp
+    got = interior(dim, s.UBufObj.toCPU())[..., :5].astype(np.float64)
+    ref = interior(dim, o.U)[..., :5]
+    assert np.all(np.isfinite(ref)) and np.all(np.isfinite(got))
+    err = rel_l1(got, ref)
+    tol = 1e-5 if mode == 'fp32' else 1e-12
+    print(f'{name} {mode}: rel L1 {err}, vacuum {seen_vac}, rho clamps {seen_rho_clamp}, P clamps {seen_p_clamp}')
+    assert np.all(err <= tol), err
+    assert abs(s.t - o.t) <= max(1e-12, tol) * abs(o.t)

@@ -323,3 +323,34 @@ def test_kh_fp32_fast_vs_fp64_oracle():
               real='float', fastMath=True)
     s, o = _pair(_kh(0), 50, **kw)
     _check(s, o, 2, TOL32, 50, 'kh-fp32-fast')
+
+
+@pytest.mark.parametrize('plm,n', [('plm cons', (128, 96)), ('ppm', (90, 70)), (None, (80, 64))])
+def test_glm_maxwell_2d_fp32_vs_fp64_oracle(plm, n):
+    """FP32 GLM-Maxwell (the serving mix's third kind: periodic, RK2-TVD,
+    sigma = 0) against the FP64 oracle from the same float inputs"""
+    from gpu_util import pair
+    kw = dict(dim=2, gridSize=n, eqn='glm_maxwell', usePLM=plm, slopeLimiter='minmod',
+              integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic', real='float', fastMath=True)
+    s, o = pair(_glm(14, 0.), **kw)
+    s.update(40)
+    o.update(40)
+    got = _interior(2, s.UBufObj.toCPU()).astype(np.float64)
+    ref = _interior(2, o.U)
+    err = rel_l1(got[..., :8], ref[..., :8])
+    print('glm fp32', plm, err)
+    assert np.all(err <= TOL32), err
+    assert abs(s.t - o.t) <= TOL32 * o.t
+
+
+@pytest.mark.parametrize('dim,n', [(2, (52, 45)), (1, 260)])
+def test_glm_maxwell_ppm_fast_fp64(dim, n):
+    """'ppm' with GLM-Maxwell in the FAST FP64 kernels (<= 1e-12)"""
+    kw = dict(dim=dim, gridSize=n, eqn='glm_maxwell', usePLM='ppm', slopeLimiter='van Leer',
+              integrator='Runge-Kutta 3, TVD', cfl=.4, boundary='mirror' if dim == 2 else 'freeflow', fastMath=True)
+    s, o = _pair(_glm(15, .5), 25, **kw)
+    s.update(25)
+    o.update(25)
+    err = rel_l1(_interior(dim, s.UBufObj.toCPU())[..., :8], _interior(dim, o.U)[..., :8])
+    print('glm ppm fast', dim, err)
+    assert np.all(err <= TOL64), err

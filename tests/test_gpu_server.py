@@ -119,3 +119,31 @@ def test_non_finite_flagged():
             out[rid] = srv.result(rid)
     assert out[ig][2] == 0 and np.isfinite(out[ig][0]).all()
     assert out[ib][2] == 1
+
+
+def test_served_fp32_results_against_fp64_oracle():
+    """every served FP32 result (1D Euler, 2D Euler and 2D GLM-Maxwell, as in
+    the serving mix) against the FP64 oracle run from the same float inputs:
+    <= 1e-5 relative L1 per conserved variable, and the same time"""
+    import pyoracle
+    from conftest import rel_l1
+    from gpu_util import interior
+    from hydro._cfg import make_cfg
+    from hydro.serve import serve
+    reqs = _small_mix(9, seed=21, real='float', fast=True)
+    for r in reqs:
+        r.steps = max(r.steps, 25)
+    _, _, served = serve(reqs, budget_bytes=256 << 20, steps_per_tick=6, keep_results=True, closed_loop=True)
+    for i, r in enumerate(reqs):
+        U, t, status = served[i]
+        assert status == 0
+        o = pyoracle.Oracle(make_cfg(**dict(r.kw, precision='double', fast_math=False)))
+        o.set_U(r.initial_state().astype(np.float64))
+        o.boundary()
+        o.update(r.steps)
+        q = 5 if r.kind != 'glm2d' else 8
+        dim = r.kw['dim']
+        err = rel_l1(interior(dim, U)[..., :q].astype(np.float64), interior(dim, o.U)[..., :q])
+        print(r.kind, r.kw['size'], r.steps, err.max())
+        assert np.all(err <= 1e-5), (i, r.kind, err)
+        assert abs(t - o.t) <= 1e-5 * o.t

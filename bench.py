@@ -10,6 +10,9 @@ Beside it on the same JSON line:
   roofline      the stage kernel: algorithmic bytes per launch / average launch
                 duration (HIP events on the solver stream around the timed
                 region, which holds only stage launches, over their count)
+  hbm_regime    the same KH problem on 4096^2 (1.07 GB of FP64 state, beyond the
+                256 MB last-level cache), FP64 and FP32: the roofline fraction
+                with the stage kernel's traffic going to HBM
   fp32          the same KH problem in the FP32 bulk form, and its relative L1
                 difference per conserved variable from the FP64 result
   fluxes        the same KH problem with the HLL, Rusanov, HLLC and Marquina
@@ -28,7 +31,8 @@ Beside it on the same JSON line:
                 solves at 80% of the measured closed-loop capacity: solves/s,
                 p50 / p99 latency, against one-at-a-time plain solves
   cpu_baseline  the CPU oracle (FP64 reference kernel sequence, OpenMP) on a
-                bounded sample of the headline workload
+                bounded sample of the headline workload, with the host's CPU
+                model, logical / affinity CPUs and cgroup quota
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20]
 """
@@ -76,7 +80,8 @@ def cpu_baseline(seconds=10.0):
     sys.path.insert(0, os.path.join(REPO, 'oracle'))
     import pyoracle
     from hydro.init import euler as init_euler
-    threads = min(16, os.cpu_count() or 1)
+    host = host_cpus()
+    threads = host['threads_used']
     pyoracle.set_threads(threads)
     o = pyoracle.Oracle(dim=2, size=(N, N), mins=(-.5, -.5), maxs=(.5, .5), plm='plm cons',
                         slope_limiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4,
@@ -95,17 +100,39 @@ def cpu_baseline(seconds=10.0):
         if el >= seconds and steps >= 2:
             break
     rate = N * N * steps / el / 1e9
-    cpu = ''
+    return {'value': rate, 'unit': 'Gcell-updates/s', 'cores': threads, 'kind': 'port',
+            'sample': f'{steps} RK2-TVD steps of the same KH 1024^2 FP64 input ({el:.1f} s), '
+                      f'oracle = reference kernel sequence, AoS, unfused, OpenMP on {threads} threads',
+            'host': host}
+
+
+def host_cpus():
+    """The host's CPUs as the process sees them: the model, the logical CPUs
+    of the machine, those this process may run on (affinity), the cgroup CPU
+    quota, and the threads the CPU baseline uses: the CPU share of one GPU on a
+    shared GPU host (16), or fewer when the process may use fewer."""
+    model = ''
     try:
         for line in open('/proc/cpuinfo'):
             if line.startswith('model name'):
-                cpu = line.split(':', 1)[1].strip()
+                model = line.split(':', 1)[1].strip()
                 break
     except OSError:
         pass
-    return {'value': rate, 'unit': 'Gcell-updates/s', 'cores': threads, 'kind': 'port',
-            'sample': f'{steps} RK2-TVD steps of the same KH 1024^2 FP64 input ({el:.1f} s), '
-                      f'oracle = reference kernel sequence, AoS, unfused, OpenMP; cpu: {cpu}'}
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else os.cpu_count()
+    quota = None
+    try:
+        q, per = open('/sys/fs/cgroup/cpu.max').read().split()[:2]
+        if q != 'max':
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    share = 16
+    limit = min(x for x in (affinity, quota, share) if x)
+    return {'cpu_model': model, 'logical_cpus': os.cpu_count(), 'affinity_cpus': affinity,
+            'cgroup_cpu_quota': quota, 'threads_used': max(1, int(limit)),
+            'note': 'threads = min(affinity, cgroup quota, 16): 16 host threads is the CPU share of one GPU '
+                    'on the shared GPU host this runs on'}
 
 
 def kh_solver(n, real, fast, rows=0, flux='roe', plm='plm cons'):
@@ -130,6 +157,28 @@ def timed_steps(s, warmup, steps):
     kt = s.timeUpdates(steps, perKernel=False)     # events on the solver stream around update(steps)
     s.sync()
     return time.perf_counter() - t0, kt.total_ms
+
+
+def hbm_leg(n, steps, warmup, fast, no_fp32):
+    """The headline workload on a grid whose state does not fit the 256 MB
+    last-level cache (4096^2: U^n and U^1 are 1.07 GB in FP64), so that the
+    stage kernel's traffic goes to HBM: Gcell-updates/s and the stage
+    kernel's roofline fraction, FP64 and FP32 bulk"""
+    res = {'workload': f'2D Euler Kelvin-Helmholtz {n}x{n} periodic, Roe + plm cons/minmod, RK2-TVD, CFL 0.4',
+           'cells': n * n, 'steps': steps}
+    for real in (('double',) if no_fp32 else ('double', 'float')):
+        s = kh_solver(n, real, fast)
+        wall, ev = timed_steps(s, warmup, steps)
+        w = 8 if real == 'double' else 4
+        avg = ev / (2 * steps)
+        alg = algorithmic_bytes_per_cell_update(w, 5) / 2 * n * n
+        res['f64' if real == 'double' else 'f32'] = {
+            'value': n * n * steps / wall / 1e9, 'unit': 'Gcell-updates/s', 'avg_launch_ms': avg,
+            'rows_per_wave': s.getLaunch()[0],
+            'roofline': {'bound': 'hbm', 'achieved': alg / (avg * 1e-3) / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                         'frac': alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 'algorithmic_bytes_per_launch': alg}}
+        del s
+    return res
 
 
 def batched_leg(count, steps, seed=0, accuracy_grids=8):
@@ -301,6 +350,9 @@ def main():
     ap.add_argument('--no-batch', action='store_true', help='skip the batched leg')
     ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
     ap.add_argument('--no-3d', action='store_true', help='skip the 3D blast leg')
+    ap.add_argument('--no-hbm', action='store_true', help='skip the HBM-regime (4096^2) KH leg')
+    ap.add_argument('--size-hbm', type=int, default=4096)
+    ap.add_argument('--steps-hbm', type=int, default=50)
     ap.add_argument('--size3d', type=int, default=256)
     ap.add_argument('--steps3d', type=int, default=200)
     ap.add_argument('--exact-math', action='store_true',
@@ -344,11 +396,15 @@ def main():
     bytes_per_launch = bytes_per_update / 2 * cells      # two stage launches per step
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
 
-    traffic = None
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes of this same command (counters cannot be read inside the run)
+    traffic, traffic_src = None, None
     tpath = os.path.join(REPO, 'profiles', 'stage_kernel_traffic.json')
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(f'kh{n}_{args.real}')
+            tj = json.load(open(tpath))
+            traffic = tj.get(f'kh{n}_{args.real}')
+            traffic_src = tj.get('source')
         except (OSError, ValueError):
             traffic = None
 
@@ -374,7 +430,7 @@ def main():
         },
         'roofline': {
             'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-            'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
+            'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic, 'traffic_source': traffic_src,
             'kernel': f'hydro::stage_kernel<{args.real}, Euler, 2, plm cons, minmod>',
             'avg_launch_ms': avg_ms,
             'algorithmic_bytes_per_launch': bytes_per_launch,
@@ -383,6 +439,8 @@ def main():
     }
     del s
 
+    if not args.no_hbm:
+        out['hbm_regime'] = hbm_leg(args.size_hbm, args.steps_hbm, args.warmup, fast, args.no_fp32)
     if not args.no_fp32 and args.real == 'double':
         s32 = kh_solver(n, 'float', True)
         wall32, ev32 = timed_steps(s32, args.warmup, args.steps)

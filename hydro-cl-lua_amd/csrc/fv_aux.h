@@ -18,7 +18,7 @@ struct GridArgs {
 	long ncell;      // interior cells (planes are dense)
 	int dim;
 	double dx[3];
-	double gamma, rhoMin, PMin, divPhi, divPsi;
+	EqnParD pd;
 	DevScalars *dev;
 	int slot;
 };
@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(256) dt_kernel(const GridArgs<R> g) {
 		for (int j = 0; j < Q; ++j) u[j] = (double)g.U[j * g.ps + c];
 #pragma unroll
 		for (int j = 0; j < E::NSTAT; ++j) u[Q + j] = (double)g.stat[j * g.ps + c];
-		double d = E::template dt_cell<DIM>(g.gamma, g.rhoMin, g.PMin, g.dx, u, g.divPhi, g.divPsi);
+		double d = E::template dt_cell<DIM>(g.pd, g.dx, u);
 		m = fmin(m, d);
 	}
 	m = wave_min(m);
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) constrain_kernel(const GridArgs<R> g) {
 		double u[Q];
 #pragma unroll
 		for (int j = 0; j < Q; ++j) u[j] = (double)g.U[j * g.ps + c];
-		E::constrain(g.gamma, g.rhoMin, g.PMin, u);
+		E::constrain(g.pd.gamma, g.pd.rhoMin, g.pd.PMin, u);
 #pragma unroll
 		for (int j = 0; j < Q; ++j) g.U[j * g.ps + c] = R(u[j]);
 	}

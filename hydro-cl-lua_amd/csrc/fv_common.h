@@ -19,10 +19,12 @@ constexpr int FLUX_ROE_ENTROPY_FIX = 16;
 constexpr int STRIP_OUT = WAVE;    // output columns per wave strip with edge passes
 // Strips whose edge interfaces come from an edge pass output 64 columns; the
 // others load a 2-cell halo in lanes 0-1 and 62-63 and output 60: 1D strips
-// (a single row, too short to repay the pass) and GLM-Maxwell (12 state reals
-// per cell make the pass's gathers cost more than the halo lanes; measured).
+// (a single row, too short to repay the pass) and equations with a source
+// term, which reads the x neighbours' static fields (GLM-Maxwell: 12 state
+// reals per cell make the pass's gathers cost more than the halo lanes,
+// measured; shallow water: the depth).
 __host__ __device__ constexpr int strip_out(bool edges) { return edges ? STRIP_OUT : WAVE - 2 * NG; }
-__host__ __device__ constexpr bool strip_edges(int dim, bool glm) { return dim >= 2 && !glm; }
+__host__ __device__ constexpr bool strip_edges(int dim, bool source) { return dim >= 2 && !source; }
 constexpr int EDGE_ROWS = WAVE / 2;// rows per wave item at most (2D): the strip-edge
                                    // interfaces of each row take a lane per side
 constexpr int EDGE_PLANES = 16;    // planes per block chunk at most (3D)
@@ -60,7 +62,7 @@ struct GridDesc {
 	R *U[4];                  // stage states U^(0..order-1); U[0] holds the solver state
 	R *D[4];                  // stored stage derivatives (tables whose later rows use them)
 	R *ping;                  // forward-Euler output (swapped with U[0] after a step)
-	const R *stat;            // static (non-integrated) planes, GLM-Maxwell only
+	const R *stat;            // static (non-integrated) planes: GLM-Maxwell, shallow water
 	int n[3];                 // interior cells
 	long ps;                  // plane stride (reals)
 	long step1;               // row stride (= n[0])
@@ -72,6 +74,7 @@ struct GridDesc {
 	R aovFast[3];             // FAST kernels: areaOverVol where flux and derivative are on, else 0
 	R dx[3];
 	R invdx[3];               // 1 / dx in the stored precision (glm-maxwell.cl:351)
+	R inv2dx[3];              // 1 / (2 dx) in the stored precision (makepartial.lua:105-108)
 	double dxD[3];
 	double cfl;
 	int stripsX, stripsY, rowsPerWave;   // 3D: stripsY counts blocks of Block3::W rows, rowsPerWave planes per chunk
@@ -81,6 +84,14 @@ struct GridDesc {
 	DevScalars *dev;          // dt slots and time of this grid
 	int stepsLeft;            // batch: steps still to run in the current tick
 	int stripsZ;              // 3D: plane chunks of rowsPerWave planes
+};
+
+// Equation parameters in double: constrainU and the per-cell dt run in double
+// in every stored precision.
+struct EqnParD {
+	double gamma, rhoMin, PMin;   // Euler (euler.lua:192-197)
+	double divPhi, divPsi;        // GLM-Maxwell (glm-maxwell.lua:110-111)
+	double gravity;               // shallow water (shallow-water.lua:167)
 };
 
 // Uniform for one launch: the stage's row of the Shu-Osher table (as buffer
@@ -106,9 +117,9 @@ struct StageArgs {
 	int stepInTick;           // batch: skip grids with stepsLeft <= stepInTick
 	// equation parameters in the stencil type and in double (for the FP64
 	// accumulation, constrainU and dt work of the FP32 form)
-	R gamma, rhoMin, PMin, divPhi, divPsi;
+	R gamma, rhoMin, PMin, divPhi, divPsi, gravity;
 	R roeFixFactor;           // Roe, Euler: Harten's entropy fix factor (roe.cl:83-106)
-	double gammaD, rhoMinD, PMinD, divPhiD, divPsiD;
+	EqnParD pd;
 	int limiter;              // flux or slope limiter index (runtime switch form)
 	int xcdRemap;
 	int nBlocks;

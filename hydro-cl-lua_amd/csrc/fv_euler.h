@@ -312,6 +312,100 @@ struct EulerT {
 			F[4] = (U[4] + W.P) * vn;
 		}
 
+		// The Roe flux F = (F(UL) + F(UR)) / 2 - R |Lambda| L (UR - UL) / 2
+		// (roe.cl:17-163 without a flux limiter) of an interface whose states
+		// are regular: both densities at least max(1e-5, rhoMin) and the Roe
+		// average's h = hTotal - v^2/2 at least 1e-5, so that none of
+		// eigen_forInterface's vacuum and cold branches (euler.cl:357-430) and
+		// no identity transform (euler.cl:432-542) applies. FAST policy only:
+		// the same function of the two states as the reference's, with its
+		// algebra collected:
+		//   Roe weight of the left state  wl = 1 / (1 + sqrt(rhoR / rhoL)),
+		//   v~ = vR + wl (vL - vR), hTotal~ likewise,
+		//   L dU through dp = g1 (dE - v~.dm + v~^2/2 drho) and
+		//   c (vn drho - dm_n): y0,4 = (dp +- c (vn drho - dm_n)) / (2 c^2),
+		//   y1 = drho - dp / c^2, tangential y = dm_t - v_t drho,
+		//   R in the matching factored form,
+		// c and 1 / c from one reciprocal square root. Returns false, with F
+		// unwritten, when any lane of the wave is not regular: the caller then
+		// takes the general form, which has every branch.
+		template <int S>
+		__device__ static __forceinline__ bool roe_regular(const Par &p, const R *UL, const R *UR, R *F) {
+			static_assert(FAST, "the collected Roe flux belongs to the FAST policy");
+			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
+			const R rhoFloor = fmax(R(1e-5), p.rhoMin);
+			if (!__all(UL[0] >= rhoFloor && UR[0] >= rhoFloor)) return false;   // NaN: not regular
+			const R iL = M::rcp(UL[0]), iR = M::rcp(UR[0]);
+			R vL[3], vR[3];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) {
+				vL[i] = i < NVEL ? UL[1 + i] * iL : R(0);
+				vR[i] = i < NVEL ? UR[1 + i] * iR : R(0);
+			}
+			const R PL = p.g1 * (UL[4] - R(.5) * dot3(UL + 1, vL));
+			const R PR = p.g1 * (UR[4] - R(.5) * dot3(UR + 1, vR));
+			const R EPL = UL[4] + PL, EPR = UR[4] + PR;
+			const R HL = EPL * iL, HR = EPR * iR;
+			const R wl = M::rcp(R(1) + M::sqrt_pos(UR[0] * iL));
+			R v[3];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) v[i] = i < NVEL ? vR[i] + wl * (vL[i] - vR[i]) : R(0);
+			const R H = HR + wl * (HL - HR);
+			const R hq = R(.5) * lenSq3(v);
+			const R h = H - hq;
+			if (!__all(h >= R(1e-5))) return false;
+			// c = sqrt(g1 h) and 1 / c: two Goldschmidt steps from v_rsq
+			const R c2 = p.g1 * h;
+			R s, hh;
+			M::sqrt_and_half_rsqrt(c2, s, hh);
+			const R c = s, ic2 = R(4) * hh * hh;   // 1 / c^2
+			// L (UR - UL)
+			R X[5];
+#pragma unroll
+			for (int j = 0; j < 5; ++j) X[j] = UR[j] - UL[j];
+			R t = X[4] + hq * X[0];
+#pragma unroll
+			for (int i = 0; i < 3; ++i)
+				if (i < NVEL) t = t - v[i] * X[1 + i];
+			const R dp = p.g1 * t;
+			const R vn = v[A];
+			const R cB = c * (vn * X[0] - X[1 + A]);
+			const R hic2 = R(.5) * ic2;
+			const R y0 = (dp + cB) * hic2, y4 = (dp - cB) * hic2;
+			const R y1 = X[0] - dp * ic2;
+			const R yB = B < NVEL ? X[1 + B] - v[B] * X[0] : R(0);
+			const R yC = C < NVEL ? X[1 + C] - v[C] * X[0] : R(0);
+			// |Lambda| L dU
+			const R an = fabs(vn);
+			const R f0 = fabs(vn - c) * y0, f4 = fabs(vn + c) * y4;
+			const R f1 = an * y1, fB = an * yB, fC = an * yC;
+			// R (|Lambda| L dU)
+			const R z0 = f0 + f1 + f4;
+			const R d40 = f4 - f0;
+			R zm[3];
+			zm[A] = vn * z0 + c * d40;
+			zm[B] = B < NVEL ? v[B] * z0 + fB : R(0);
+			zm[C] = C < NVEL ? v[C] * z0 + fC : R(0);
+			R z4 = H * (f0 + f4) + (c * vn) * d40 + hq * f1;
+			if constexpr (B < NVEL) z4 = z4 + v[B] * fB;
+			if constexpr (C < NVEL) z4 = z4 + v[C] * fC;
+			// (F(UL) + F(UR) - z) / 2
+			const R vnL = vL[A], vnR = vR[A];
+			F[0] = R(.5) * ((UL[1 + A] + UR[1 + A]) - z0);
+#pragma unroll
+			for (int i = 0; i < 3; ++i) {
+				if (i < NVEL || i == A) {
+					R fs = UL[1 + i] * vnL + UR[1 + i] * vnR;
+					if (i == A) fs = fs + (PL + PR);
+					F[1 + i] = R(.5) * (fs - zm[i]);
+				} else {
+					F[1 + i] = R(0);
+				}
+			}
+			F[4] = R(.5) * ((EPL * vnL + EPR * vnR) - z4);
+			return true;
+		}
+
 		// eigenWaveCode (euler.lua:309-325)
 		template <int S>
 		__device__ static __forceinline__ void waves(const Par &, const Eig &e, R *l) {
@@ -344,10 +438,9 @@ struct EulerT {
 
 		// calcDTCell (calcDT.cl:38-73, euler.lua:346-373) in double
 		template <int DIM>
-		__device__ static __forceinline__ double dt_cell(double gamma, double rhoMin, double PMin,
-			const double *dx, const double *U, double, double)
-		{
+		__device__ static __forceinline__ double dt_cell(const EqnParD &pd, const double *dx, const double *U) {
 			using MD = Math<double, FAST>;
+			const double gamma = pd.gamma, rhoMin = pd.rhoMin, PMin = pd.PMin;
 			double inv = MD::rcp(U[0]);
 			double lenSq = lenSq3(U + 1);
 			double EKin = U[0] < rhoMin ? 0. : MD::div_inv(.5 * lenSq, U[0], inv);
@@ -371,7 +464,8 @@ struct EulerT {
 		}
 
 		template <int DIM>
-		__device__ static __forceinline__ void source(const Par &, const R *, const R *, const R *, const R *, R *) {}
+		__device__ static __forceinline__ void source(const Par &, const GridDesc<R> &, const R *, const R *,
+			const R *, R *) {}
 
 		// mirror reflects the normal momentum component
 		__device__ static __forceinline__ bool reflects(int field, int axis) { return field == 1 + axis; }

@@ -1,8 +1,8 @@
-// FP64 kernels of the EXACT policy, the dispatch and the auxiliary kernels.
+// FP64 dispatch of the stage kernels by policy and dimension, and the FP64
+// auxiliary kernels (dt, constrainU, conserved totals, statistics, dt slots).
 // Built with -ffp-contract=off so the arithmetic is the reference's expression
 // order, operation for operation.
 #define REAL double
-#define STAGE_EXACT
 #define AUX_KERNELS
 #include "fv_kernels.inc"
 

@@ -207,10 +207,12 @@ struct Eqn {
 	// cons_t, em / ep the sqrt_1_eps of its -1 / +1 neighbours per axis. The
 	// reference's second gradient loop stores into grad_1_mu (so grad_1_mu ends
 	// as the sqrt_1_eps gradient and grad_1_eps stays zero); kept as is.
+	static constexpr int SRC_FIELD = 10;   // the source reads the neighbours' sqrt_1_eps
 	template <int DIM>
-	__device__ static __forceinline__ void source(const Par &p, const R *invdx, const R *U, const R *em,
+	__device__ static __forceinline__ void source(const Par &p, const GridDesc<R> &gd, const R *U, const R *em,
 		const R *ep, R *d)
 	{
+		const R *invdx = gd.invdx;
 		R c = U[10] * U[10] * U[9];
 		d[0] = d[0] - U[0] * c;
 		d[1] = d[1] - U[1] * c;
@@ -239,10 +241,9 @@ struct Eqn {
 
 	// calcDTCell with the GLM wave bounds (glm-maxwell.lua:216-245)
 	template <int DIM>
-	__device__ static __forceinline__ double dt_cell(double, double, double, const double *dx, const double *U,
-		double divPhi, double divPsi)
-	{
+	__device__ static __forceinline__ double dt_cell(const EqnParD &pd, const double *dx, const double *U) {
 		using MD = Math<double, FAST>;
+		const double divPhi = pd.divPhi, divPsi = pd.divPsi;
 		double dt = INFINITY;
 		for (int s = 0; s < DIM; ++s) {
 			if (dx[s] > 0.) {

@@ -39,8 +39,32 @@ static inline int fail(const char *fmt, ...) {
 
 static inline void ignore(hipError_t) {}
 
-static inline int num_states(int eqn) { return eqn == HYDRO_EQN_EULER ? 6 : eqn == HYDRO_EQN_GLM_MAXWELL ? 12 : -1; }
-static inline int num_int_states(int eqn) { return eqn == HYDRO_EQN_EULER ? 5 : eqn == HYDRO_EQN_GLM_MAXWELL ? 8 : -1; }
+static inline int num_states(int eqn) {
+	return eqn == HYDRO_EQN_EULER ? 6 : eqn == HYDRO_EQN_GLM_MAXWELL ? 12 : eqn == HYDRO_EQN_SHALLOW_WATER ? 5 : -1;
+}
+static inline int num_int_states(int eqn) {
+	return eqn == HYDRO_EQN_EULER ? 5 : eqn == HYDRO_EQN_GLM_MAXWELL ? 8 : eqn == HYDRO_EQN_SHALLOW_WATER ? 4 : -1;
+}
+// static (non-integrated) planes on the device: GLM-Maxwell's rhoCharge,
+// sigma, sqrt_1_eps, sqrt_1_mu; the shallow water depth (Euler's ePot stays on
+// the host)
+static inline int num_static_states(int eqn) {
+	return eqn == HYDRO_EQN_GLM_MAXWELL ? 4 : eqn == HYDRO_EQN_SHALLOW_WATER ? 1 : 0;
+}
+// equations with an addSource step, whose strips load a halo instead of
+// taking an edge pass (the source reads the x neighbours' static fields)
+static inline bool eqn_has_source(int eqn) { return eqn == HYDRO_EQN_GLM_MAXWELL || eqn == HYDRO_EQN_SHALLOW_WATER; }
+
+static inline EqnParD eqn_par_d(const hydro_cfg &c) {
+	EqnParD p;
+	p.gamma = c.gamma;
+	p.rhoMin = c.rho_min;
+	p.PMin = c.P_min;
+	p.divPhi = c.div_phi_wavespeed;
+	p.divPsi = c.div_psi_wavespeed;
+	p.gravity = c.gravity;
+	return p;
+}
 
 static inline void cfg_default(hydro_cfg *c) {
 	std::memset(c, 0, sizeof *c);
@@ -58,6 +82,7 @@ static inline void cfg_default(hydro_cfg *c) {
 	c->integrator = HYDRO_INT_FORWARD_EULER;
 	c->cfl = .5;
 	c->roe_entropy_fix_factor = .2;
+	c->gravity = 1;
 	for (int i = 0; i < 6; ++i) c->boundary[i] = HYDRO_BC_FREEFLOW;
 	c->precision = HYDRO_REAL_F64;
 	c->gamma = 7. / 5.;
@@ -155,13 +180,17 @@ static int validate(const hydro_cfg *c) {
 	if (!c) return fail("cfg is NULL");
 	if (c->dim < 1 || c->dim > 3) return fail("dim %d not supported (1, 2 or 3)", c->dim);
 	if (c->dim == 3 && c->eqn != HYDRO_EQN_EULER) return fail("3D is built for the Euler equations only");
+	if (c->eqn == HYDRO_EQN_SHALLOW_WATER && !(c->gravity > 0)) return fail("gravity must be > 0");
 	for (int i = 0; i < c->dim; ++i)
 		if (c->size[i] < 1) return fail("size[%d] = %d must be >= 1", i, c->size[i]);
-	if (c->eqn != HYDRO_EQN_EULER && c->eqn != HYDRO_EQN_GLM_MAXWELL) return fail("eqn %d not supported", c->eqn);
+	if (c->eqn != HYDRO_EQN_EULER && c->eqn != HYDRO_EQN_GLM_MAXWELL && c->eqn != HYDRO_EQN_SHALLOW_WATER)
+		return fail("eqn %d not supported", c->eqn);
 	if (c->flux < HYDRO_FLUX_ROE || c->flux > HYDRO_FLUX_MARQUINA)
 		return fail("flux %d not supported", c->flux);
 	if (c->flux != HYDRO_FLUX_ROE && c->eqn != HYDRO_EQN_EULER)
 		return fail("the HLL, Rusanov, HLLC and Marquina fluxes take the Euler equations");
+	if (c->eqn == HYDRO_EQN_SHALLOW_WATER && c->plm > HYDRO_PLM_CONS)
+		return fail("the shallow water equations take 'plm cons' or no reconstruction");
 	if (c->roe_entropy_fix && (c->flux != HYDRO_FLUX_ROE || c->eqn != HYDRO_EQN_EULER))
 		return fail("the entropy fix belongs to the Roe flux of the Euler equations (roe.cl:83-106)");
 	if (c->roe_entropy_fix && !(c->roe_entropy_fix_factor > 0))
@@ -214,7 +243,7 @@ static long device_cus() {
 // divide the depth evenly, each with its 1-3 plane warm-up).
 static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 	if (cfg.dim < 2) return 1;
-	const int so = strip_out(strip_edges(cfg.dim, cfg.eqn == HYDRO_EQN_GLM_MAXWELL));
+	const int so = strip_out(strip_edges(cfg.dim, eqn_has_source(cfg.eqn)));
 	long stripsX = (nx + so - 1) / so;
 	int warm = cfg.plm ? 2 : (cfg.flux_limiter ? 3 : 1);
 	if (cfg.dim == 3) {
@@ -249,7 +278,7 @@ static void grid_init(Grid &g, const hydro_cfg &cfg) {
 	g.dim = cfg.dim;
 	g.Q = num_int_states(cfg.eqn);
 	g.S = num_states(cfg.eqn);
-	g.NSTAT = cfg.eqn == HYDRO_EQN_GLM_MAXWELL ? 4 : 0;
+	g.NSTAT = num_static_states(cfg.eqn);
 	g.ncell = 1;
 	for (int i = 0; i < 3; ++i) {
 		g.n[i] = i < g.dim ? cfg.size[i] : 1;
@@ -340,6 +369,7 @@ static void grid_desc(const Grid &g, GridDesc<R> &d) {
 		dxr[i] = R((c.maxs[i] - c.mins[i]) / (double)g.n[i]);
 		d.dx[i] = dxr[i];
 		d.invdx[i] = R(1) / dxr[i];
+		d.inv2dx[i] = R(1) / (dxr[i] * R(2));
 		d.dxD[i] = (c.maxs[i] - c.mins[i]) / (double)g.n[i];
 	}
 	R volume = R(1);
@@ -360,7 +390,7 @@ static void grid_desc(const Grid &g, GridDesc<R> &d) {
 	}
 	d.cfl = c.cfl;
 	d.rowsPerWave = g.rowsPerWave;
-	const int so = strip_out(strip_edges(g.dim, g.cfg.eqn == HYDRO_EQN_GLM_MAXWELL));
+	const int so = strip_out(strip_edges(g.dim, eqn_has_source(g.cfg.eqn)));
 	d.stripsX = (g.n[0] + so - 1) / so;
 	if (g.dim == 3) {
 		d.stripsY = (g.n[1] + Block3<R>::W - 1) / Block3<R>::W;
@@ -409,11 +439,8 @@ static void stage_args(const Grid &g, int s, StageArgs<R> &a) {
 	a.PMin = R(c.P_min);
 	a.divPhi = R(c.div_phi_wavespeed);
 	a.divPsi = R(c.div_psi_wavespeed);
-	a.gammaD = c.gamma;
-	a.rhoMinD = c.rho_min;
-	a.PMinD = c.P_min;
-	a.divPhiD = c.div_phi_wavespeed;
-	a.divPsiD = c.div_psi_wavespeed;
+	a.gravity = R(c.gravity);
+	a.pd = eqn_par_d(c);
 	a.roeFixFactor = R(c.roe_entropy_fix_factor);
 	a.limiter = c.plm ? c.slope_limiter : c.flux_limiter;
 	a.resetSlot = -1;
@@ -467,7 +494,7 @@ static HostMap host_map(int c, int n, int kmin, int kmax, int faceMin) {
 }
 
 static bool reflects(int eqn, int field, int axis) {
-	if (eqn == HYDRO_EQN_EULER) return field == 1 + axis;
+	if (eqn == HYDRO_EQN_EULER || eqn == HYDRO_EQN_SHALLOW_WATER) return field == 1 + axis;
 	return field == axis || field == 3 + axis;
 }
 
@@ -589,11 +616,7 @@ static GridArgs<R> grid_args(const Grid &g, int slot) {
 	a.ncell = g.ncell;
 	a.dim = g.dim;
 	for (int i = 0; i < 3; ++i) a.dx[i] = (g.cfg.maxs[i] - g.cfg.mins[i]) / (double)g.n[i];
-	a.gamma = g.cfg.gamma;
-	a.rhoMin = g.cfg.rho_min;
-	a.PMin = g.cfg.P_min;
-	a.divPhi = g.cfg.div_phi_wavespeed;
-	a.divPsi = g.cfg.div_psi_wavespeed;
+	a.pd = eqn_par_d(g.cfg);
 	a.dev = g.dev;
 	a.slot = slot;
 	return a;

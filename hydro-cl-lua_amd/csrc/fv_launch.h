@@ -1,5 +1,5 @@
 // fv_launch.h — host-side launchers of the device kernels, one set per stored
-// precision (defined in fv_f64.hip, fv_f64_fast.hip and fv_f32.hip).
+// precision (defined in fv_f64.hip, fv_f32.hip and the fv_stage_tu.hip builds).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,9 +41,10 @@ template <typename R> struct GridArgs;
 // batched launch over a.grids.
 template <typename R>
 hipError_t launch_stage(const StageKey &k, const StageArgs<R> &a, dim3 grid, dim3 block, hipStream_t s);
-// the same for one math policy (launch_stage picks by k.fast)
-template <typename R, bool FAST>
-hipError_t launch_stage_policy(const StageKey &k, const StageArgs<R> &a, dim3 grid, dim3 block, hipStream_t s);
+// the same for one math policy and dimension (launch_stage picks by k.fast and
+// k.dim); each is defined in a translation unit of its own
+template <typename R, bool FAST, int DIM>
+hipError_t launch_stage_dim(const StageKey &k, const StageArgs<R> &a, dim3 grid, dim3 block, hipStream_t s);
 
 template <typename R>
 hipError_t launch_dt(int eqn, int dim, int fast, const GridArgs<R> &g, hipStream_t s);

@@ -70,6 +70,19 @@ struct Math<double, true> {
 		double d = __builtin_fma(-s, s, x);
 		return __builtin_fma(d, h, s);
 	}
+	// s = sqrt(x) and hh = 1 / (2 sqrt(x)) of a positive, normal x: two
+	// Goldschmidt steps from v_rsq_f64 (each squares the relative error)
+	__device__ static __forceinline__ void sqrt_and_half_rsqrt(double x, double &s, double &hh) {
+		double y = __builtin_amdgcn_rsq(x);
+		s = x * y;
+		hh = .5 * y;
+		double r = __builtin_fma(-s, hh, .5);
+		s = __builtin_fma(s, r, s);
+		hh = __builtin_fma(hh, r, hh);
+		r = __builtin_fma(-s, hh, .5);
+		s = __builtin_fma(s, r, s);
+		hh = __builtin_fma(hh, r, hh);
+	}
 	__device__ static __forceinline__ double sqrt_(double x) {
 		double y = __builtin_amdgcn_rsq(x);
 		double s = x * y, h = .5 * y;
@@ -110,6 +123,11 @@ struct Math<float, true> {
 	__device__ static __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
 	__device__ static __forceinline__ float sqrt_pos(float x) { return __builtin_amdgcn_sqrtf(x); }
 	__device__ static __forceinline__ float div_pos(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+	__device__ static __forceinline__ void sqrt_and_half_rsqrt(float x, float &s, float &hh) {
+		float y = __builtin_amdgcn_rsqf(x);
+		s = x * y;
+		hh = .5f * y;
+	}
 };
 
 } // namespace hydro

@@ -23,6 +23,7 @@
 #include "fv_euler.h"
 #include "fv_glm.h"
 #include "fv_limiters.h"
+#include "fv_sw.h"
 
 namespace hydro {
 
@@ -452,6 +453,10 @@ __device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell
 	R dt_dx, int lim, R *F)
 {
 	if constexpr (E::FLUXID == HYDRO_FLUX_ROE || E::FLUXID == FLUX_ROE_ENTROPY_FIX) {
+		if constexpr (E::FASTM && E::FLUXID == HYDRO_FLUX_ROE && E::Q == 5 && E::NW == 5) {
+			// the wave's interfaces are all regular: the collected form
+			if (E::template roe_regular<S>(p, UL.v, UR.v, F)) return;
+		}
 		Iface<R, E> I;
 		iface_prep<R, E, S>(p, UL, UR, I);
 		iface_flux<R, E, S, false, 0>(p, I, nullptr, nullptr, dt_dx, lim, F);
@@ -887,10 +892,11 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			if constexpr (E::SOURCE) {
 				// eqn addSource after calcDerivFromFlux (solverbase.lua:3208-3214)
 				R em[3], ep[3];
-				em[0] = Um.v[10];
-				ep[0] = Up.v[10];
-				if constexpr (DIM >= 2) { em[1] = Wm.v[10]; ep[1] = W1.v[10]; }
-				E::template source<DIM>(par, g.invdx, U.v, em, ep, deriv);
+				constexpr int SF = E::SRC_FIELD;
+				em[0] = Um.v[SF];
+				ep[0] = Up.v[SF];
+				if constexpr (DIM >= 2) { em[1] = Wm.v[SF]; ep[1] = W1.v[SF]; }
+				E::template source<DIM>(par, g, U.v, em, ep, deriv);
 			}
 
 			// ---- integrator combine (rk.lua:155-174, fe.lua:50), constrainU
@@ -928,7 +934,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 					for (int j = 0; j < Q; ++j)
 						if (E::active(j)) acc[j] = acc[j] + (double)deriv[j] * c;
 				}
-				E::constrain(a.gammaD, a.rhoMinD, a.PMinD, acc);
+				E::constrain(a.pd.gamma, a.pd.rhoMin, a.pd.PMin, acc);
 				R o[Q];
 #pragma unroll
 				for (int j = 0; j < Q; ++j) {
@@ -941,7 +947,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 					for (int j = 0; j < Q; ++j) ud[j] = (double)o[j];
 #pragma unroll
 					for (int j = Q; j < C::NV; ++j) ud[j] = (double)U.v[j];
-					double d = E::template dt_cell<DIM>(a.gammaD, a.rhoMinD, a.PMinD, g.dxD, ud, a.divPhiD, a.divPsiD);
+					double d = E::template dt_cell<DIM>(a.pd, g.dxD, ud);
 					dtLocal = fmin(dtLocal, d);
 				}
 			}

@@ -377,7 +377,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 					for (int j = 0; j < Q; ++j)
 						if (E::active(j)) acc[j] = acc[j] + (double)deriv[j] * c;
 				}
-				E::constrain(a.gammaD, a.rhoMinD, a.PMinD, acc);
+				E::constrain(a.pd.gamma, a.pd.rhoMin, a.pd.PMin, acc);
 				R o[Q];
 #pragma unroll
 				for (int j = 0; j < Q; ++j) {
@@ -390,7 +390,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 					for (int j = 0; j < Q; ++j) ud[j] = (double)o[j];
 #pragma unroll
 					for (int j = Q; j < NV; ++j) ud[j] = (double)U.v[j];
-					double d = E::template dt_cell<3>(a.gammaD, a.rhoMinD, a.PMinD, g.dxD, ud, a.divPhiD, a.divPsiD);
+					double d = E::template dt_cell<3>(a.pd, g.dxD, ud);
 					dtLocal = fmin(dtLocal, d);
 				}
 			}

@@ -357,7 +357,8 @@ static int same_kernel_config(const hydro_cfg &a, const hydro_cfg &b) {
 		&& a.rho_min == b.rho_min && a.P_min == b.P_min && a.div_phi_wavespeed == b.div_phi_wavespeed
 		&& a.div_psi_wavespeed == b.div_psi_wavespeed && a.use_fixed_dt == b.use_fixed_dt
 		&& (!a.use_fixed_dt || a.fixed_dt == b.fixed_dt) && a.roe_entropy_fix == b.roe_entropy_fix
-		&& (!a.roe_entropy_fix || a.roe_entropy_fix_factor == b.roe_entropy_fix_factor);
+		&& (!a.roe_entropy_fix || a.roe_entropy_fix_factor == b.roe_entropy_fix_factor)
+		&& a.gravity == b.gravity;
 }
 
 static void batch_drop_graph(hydro_batch *b) {
@@ -371,7 +372,7 @@ static long batch_items(const hydro_batch *b) {
 	long n = 0;
 	for (auto &g : b->grids) {
 		int rows = g.dim >= 2 ? b->rowsPerWave : 1;
-		const int so = strip_out(strip_edges(g.dim, g.cfg.eqn == HYDRO_EQN_GLM_MAXWELL));
+		const int so = strip_out(strip_edges(g.dim, eqn_has_source(g.cfg.eqn)));
 		long sx = (g.n[0] + so - 1) / so;
 		long sy = g.dim >= 2 ? (g.n[1] + rows - 1) / rows : 1;
 		n += sx * sy;

@@ -97,7 +97,8 @@ bool same_pool(const hydro_cfg &a, const hydro_cfg &b) {
 		&& a.rho_min == b.rho_min && a.P_min == b.P_min && a.div_phi_wavespeed == b.div_phi_wavespeed
 		&& a.div_psi_wavespeed == b.div_psi_wavespeed && a.use_fixed_dt == b.use_fixed_dt
 		&& (!a.use_fixed_dt || a.fixed_dt == b.fixed_dt) && a.roe_entropy_fix == b.roe_entropy_fix
-		&& (!a.roe_entropy_fix || a.roe_entropy_fix_factor == b.roe_entropy_fix_factor);
+		&& (!a.roe_entropy_fix || a.roe_entropy_fix_factor == b.roe_entropy_fix_factor)
+		&& a.gravity == b.gravity;
 }
 
 template <typename R>

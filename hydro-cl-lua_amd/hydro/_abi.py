@@ -17,7 +17,8 @@ EXPORTS = [
     'hydro_integrator_table',
     'hydro_device_count', 'hydro_create', 'hydro_destroy', 'hydro_num_cells', 'hydro_upload_U',
     'hydro_download_U', 'hydro_constrain_U', 'hydro_boundary', 'hydro_calc_dt', 'hydro_step',
-    'hydro_update', 'hydro_get_time', 'hydro_set_time', 'hydro_conserved_totals', 'hydro_var_stats', 'hydro_sync',
+    'hydro_update', 'hydro_get_time', 'hydro_set_time', 'hydro_conserved_totals',
+    'hydro_var_stats', 'hydro_sync',
     'hydro_time_updates', 'hydro_device_bytes', 'hydro_set_launch', 'hydro_get_launch',
     'hydro_batch_create', 'hydro_batch_destroy', 'hydro_batch_size', 'hydro_batch_device_bytes',
     'hydro_batch_upload_U', 'hydro_batch_download_U', 'hydro_batch_constrain_U', 'hydro_batch_update',

@@ -27,7 +27,7 @@ INTEGRATORS = [
 INTEGRATOR_STAGES = [1, 2, 2, 2, 3, 4, 4, 2, 2, 3, 4, 4]
 
 BOUNDARIES = ['periodic', 'mirror', 'freeflow', 'fixed']
-EQNS = ['euler', 'glm_maxwell']
+EQNS = ['euler', 'glm_maxwell', 'shallow_water']
 FLUXES = ['roe', 'hll', 'rusanov', 'euler-hllc', 'marquina']
 PLMS = [None, 'plm cons', 'plm prim', 'plm eig', 'plm cons with flux', 'plm athena', 'ppm',
         'plm eig prim', 'plm eig prim ref']
@@ -44,8 +44,8 @@ EULER_DISPLAY_VARS = {name: 64 + i for i, name in enumerate([
     'P', 'eInt', 'eKin', 'eTotal', 'EInt', 'EKin', 'S', 'H', 'h', 'HTotal', 'hTotal',
     'speed of sound', 'Mach number', 'v x', 'v y', 'v z'])}
 
-EQN_NUM_STATES = {'euler': 6, 'glm_maxwell': 12}
-EQN_NUM_INT_STATES = {'euler': 5, 'glm_maxwell': 8}
+EQN_NUM_STATES = {'euler': 6, 'glm_maxwell': 12, 'shallow_water': 5}
+EQN_NUM_INT_STATES = {'euler': 5, 'glm_maxwell': 8, 'shallow_water': 4}
 
 
 class HydroCfg(ctypes.Structure):
@@ -74,6 +74,7 @@ class HydroCfg(ctypes.Structure):
         ('fast_math', ctypes.c_int),
         ('roe_entropy_fix', ctypes.c_int),
         ('roe_entropy_fix_factor', ctypes.c_double),
+        ('gravity', ctypes.c_double),
     ]
 
 
@@ -94,7 +95,7 @@ def make_cfg(dim, size, mins=None, maxs=None, eqn='euler', flux='roe',
              boundary='freeflow', boundary_fixed=None, precision='double',
              gamma=7 / 5, rho_min=1e-7, P_min=1e-7,
              div_phi_wavespeed=1.0, div_psi_wavespeed=1.0, fast_math=False,
-             use_entropy_flux_fix=False, entropy_flux_fix_limiting_factor=0.2):
+             use_entropy_flux_fix=False, entropy_flux_fix_limiting_factor=0.2, gravity=1.0):
     """Builds a HydroCfg from reference-style arguments (names or indices).
 
     `boundary` is a name for all faces or a dict face -> name, like the
@@ -140,15 +141,17 @@ def make_cfg(dim, size, mins=None, maxs=None, eqn='euler', flux='roe',
     # Roe: args.useEntropyFluxFix / entropyFluxFixLimitingFactor (roe.lua:10-16)
     c.roe_entropy_fix = 1 if use_entropy_flux_fix else 0
     c.roe_entropy_fix_factor = float(entropy_flux_fix_limiting_factor)
+    # shallow water: gravity (shallow-water.lua:167)
+    c.gravity = float(gravity)
     return c
 
 
 def cfg_num_states(cfg):
-    return 6 if cfg.eqn == 0 else 12
+    return {0: 6, 1: 12, 2: 5}[cfg.eqn]
 
 
 def cfg_num_int_states(cfg):
-    return 5 if cfg.eqn == 0 else 8
+    return {0: 5, 1: 8, 2: 4}[cfg.eqn]
 
 
 def cfg_ghost_shape(cfg):

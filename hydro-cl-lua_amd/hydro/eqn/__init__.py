@@ -2,11 +2,14 @@
 name, numStates / numIntStates / numWaves, consVars / primVars / eigenVars,
 default parameters (guiVars) and the host-side cons <-> prim conversions.
 The stage arithmetic itself lives in the HIP kernels (csrc/fv_euler.h,
-csrc/fv_glm.h); these classes describe it and select it by name."""
+csrc/fv_glm.h, csrc/fv_sw.h); these classes describe it and select it by
+name."""
 from .euler import Euler
 from .glm_maxwell import GLMMaxwell
+from .shallow_water import ShallowWater
 
-EQUATIONS = {'euler': Euler, 'glm_maxwell': GLMMaxwell, 'glm-maxwell': GLMMaxwell}
+EQUATIONS = {'euler': Euler, 'glm_maxwell': GLMMaxwell, 'glm-maxwell': GLMMaxwell,
+             'shallow_water': ShallowWater, 'shallow-water': ShallowWater}
 
 
 def get(name_or_obj):

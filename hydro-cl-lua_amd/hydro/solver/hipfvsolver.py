@@ -68,7 +68,7 @@ class HipFVSolver:
                             rho_min=eqnArgs.get('rhoMin', 1e-7), P_min=eqnArgs.get('PMin', 1e-7),
                             div_phi_wavespeed=eqnArgs.get('divPhiWavespeed', 1.),
                             div_psi_wavespeed=eqnArgs.get('divPsiWavespeed', 1.),
-                            fast_math=fastMath,
+                            gravity=eqnArgs.get('gravity', 1.), fast_math=fastMath,
                             use_entropy_flux_fix=getattr(self.flux, 'useEntropyFluxFix', False),
                             entropy_flux_fix_limiting_factor=getattr(self.flux, 'entropyFluxFixLimitingFactor', .2))
         self.dim = int(dim)

@@ -24,9 +24,11 @@ extern "C" {
 #define HYDRO_MAX_STATES 12
 
 /* Equation plugins: hydro/eqn/euler.lua (cons_t = rho, m.xyz, ETotal, ePot;
- * 5 integrated, 5 waves) and hydro/eqn/glm-maxwell.lua (cons_t = D.xyz, B.xyz,
- * phi, psi, rhoCharge, sigma, sqrt_1_eps, sqrt_1_mu; 8 integrated, 8 waves). */
-enum { HYDRO_EQN_EULER = 0, HYDRO_EQN_GLM_MAXWELL = 1 };
+ * 5 integrated, 5 waves), hydro/eqn/glm-maxwell.lua (cons_t = D.xyz, B.xyz,
+ * phi, psi, rhoCharge, sigma, sqrt_1_eps, sqrt_1_mu; 8 integrated, 8 waves)
+ * and hydro/eqn/shallow-water.lua in its depthInCell = false form (cons_t =
+ * h, m.xyz, depth; 4 integrated, 4 waves; shallow-water.lua:34-86). */
+enum { HYDRO_EQN_EULER = 0, HYDRO_EQN_GLM_MAXWELL = 1, HYDRO_EQN_SHALLOW_WATER = 2 };
 
 /* Flux plugins: hydro/flux/roe.lua, hll.lua ('Davis direct bounded' wave
  * estimates, hll.lua:10), rusanov.lua, euler-hllc.lua (hllcMethod 2,
@@ -137,11 +139,13 @@ typedef struct hydro_cfg {
 	int roe_entropy_fix;     /* Roe flux, Euler: Harten's entropy fix (args.useEntropyFluxFix,
 	                          * roe.lua:10-16, roe.cl:83-106; default off) */
 	double roe_entropy_fix_factor; /* args.entropyFluxFixLimitingFactor (roe.lua:11, 0.2) */
+	double gravity;          /* shallow water: gravity (shallow-water.lua:167, default 1) */
 } hydro_cfg;
 
 /* Reference defaults: mins -1, maxs 1 (gridsolver.lua:80-81), gamma 7/5,
- * rhoMin = PMin = 1e-7 (euler.lua:192-197), cfl .5, boundaries freeflow
- * (gridsolver.lua:124), limiters donor cell, Roe, forward Euler, F64. */
+ * rhoMin = PMin = 1e-7 (euler.lua:192-197), gravity 1 (shallow-water.lua:167),
+ * cfl .5, boundaries freeflow (gridsolver.lua:124), limiters donor cell, Roe,
+ * forward Euler, F64. */
 void hydro_cfg_default(hydro_cfg *cfg);
 
 /* Last error message of the calling thread ("" if none). */

@@ -470,30 +470,174 @@ static real glm_dt_cell(const oracle_params *p, const real *U, int dim, const re
 	return dt;
 }
 
+/* ----------------------------------------------------------- shallow water */
+/* hydro/eqn/shallow-water.lua / .cl with depthInCell = false (the depth in
+ * cons_t, not integrated) and extraTermInFlux = false, Cartesian normals:
+ * cons 0 h, 1-3 m, 4 depth; prim 0 h, 1-3 v, 4 depth; eigen 0 h, 1-3 v, 4 C,
+ * 5 depth (shallow-water.lua:293-306). */
+enum { SW_H = 0, SW_C = 4, SW_DEPTH = 5 };
+
+void oracle_sw_prim_from_cons(const oracle_params *p, const double *U, double *W) {
+	/* shallow-water.cl:3-15 */
+	W[0] = U[0];
+	real inv = 1. / U[0];
+	for (int i = 0; i < 3; ++i) W[1 + i] = U[1 + i] * inv;
+	W[4] = U[4];
+}
+
+void oracle_sw_eigen_for_interface(const oracle_params *p, const double *UL, const double *UR, double *e) {
+	/* shallow-water.cl:228-265: Roe velocity with sqrt(h) weights, the
+	 * arithmetic mean height */
+	real WL[5], WR[5];
+	oracle_sw_prim_from_cons(p, UL, WL);
+	oracle_sw_prim_from_cons(p, UR, WR);
+	real sqrtHL = sqrt(WL[0]), sqrtHR = sqrt(WR[0]);
+	real invDenom = 1. / (sqrtHL + sqrtHR);
+	e[SW_H] = .5 * (UL[0] + UR[0]);
+	for (int i = 0; i < 3; ++i) e[1 + i] = WL[1 + i] * (sqrtHL * invDenom) + WR[1 + i] * (sqrtHR * invDenom);
+	e[SW_DEPTH] = .5 * (UR[4] + UL[4]);
+	real CSq = p->gravity * e[SW_H];
+	e[SW_C] = sqrt(CSq);
+}
+
+void oracle_sw_left(const oracle_params *p, const double *e, const double *X, int s, double *Y) {
+	/* shallow-water.cl:286-318 (nLen = 1; nL, n2L, n3L the Cartesian l1..l3 rows) */
+	real nLen = 1.;
+	real nLx = nl(1, 1, s), nLy = nl(1, 2, s), nLz = nl(1, 3, s);
+	real n2Lx = nl(2, 1, s), n2Ly = nl(2, 2, s), n2Lz = nl(2, 3, s);
+	real n3Lx = nl(3, 1, s), n3Ly = nl(3, 2, s), n3Lz = nl(3, 3, s);
+	real v_n = e[1 + s];
+	real invDenom = 1. / (2. * nLen * e[SW_H] * e[SW_C]);
+	real tmp1 = e[SW_C] * X[0];
+	real tmp2 = nLen * tmp1;
+	real tmp3 = v_n * X[0];
+	real tmp5 = -tmp3;
+	real tmp6 = nLz * X[3];
+	real tmp8 = nLy * X[2];
+	real tmp10 = nLx * X[1];
+	real tmp13 = e[1] * X[0];
+	real tmp15 = e[2] * X[0];
+	real tmp17 = e[3] * X[0];
+	real tmp29 = nLen * e[SW_C];
+	real tmp31 = invDenom * tmp29;
+	Y[0] = invDenom * (tmp10 + tmp8 + tmp6 + tmp5 + -tmp2);
+	Y[1] = 2. * tmp31 * (n2Lx * X[1] + n2Ly * X[2] + n2Lz * X[3] + -n2Lx * tmp13 + -n2Lz * tmp17 + -n2Ly * tmp15);
+	Y[2] = 2. * tmp31 * (n3Lx * X[1] + n3Ly * X[2] + n3Lz * X[3] + -n3Lx * tmp13 + -n3Lz * tmp17 + -n3Ly * tmp15);
+	Y[3] = invDenom * (tmp10 + tmp8 + tmp6 + tmp2 + tmp5);
+}
+
+void oracle_sw_right(const oracle_params *p, const double *e, const double *X, int s, double *Y) {
+	/* shallow-water.cl:322-362 (n2Len = n3Len = nLen = 1) */
+	real nU[3] = {nl(1, 1, s), nl(1, 2, s), nl(1, 3, s)};
+	real n2U[3] = {nl(2, 1, s), nl(2, 2, s), nl(2, 3, s)};
+	real n3U[3] = {nl(3, 1, s), nl(3, 2, s), nl(3, 3, s)};
+	real nLen = 1., n2Len = 1., n3Len = 1.;
+	real h = e[SW_H], C = e[SW_C];
+	real invDenom = 1. / C;
+	real tmp5 = n3Len * n3Len;
+	real tmp6 = n2Len * n2Len;
+	real tmp7 = tmp5 * X[0];
+	real tmp8 = C * C;
+	real tmp9 = tmp6 * tmp7;
+	real tmp10 = tmp8 * tmp9;
+	real tmp13 = X[3] * tmp5;
+	real tmp15 = tmp13 * tmp6;
+	real tmp16 = tmp15 * tmp8;
+	real tmp17 = C * X[0];
+	real tmp19 = nLen * tmp17;
+	real tmp21 = tmp19 * tmp5;
+	real tmp22 = tmp21 * tmp6;
+	real tmp24 = C * X[1];
+	real tmp26 = nLen * tmp24;
+	real tmp27 = tmp26 * tmp5;
+	real tmp28 = C * X[2];
+	real tmp30 = nLen * tmp28;
+	real tmp31 = tmp30 * tmp6;
+	real tmp32 = C * X[3];
+	real tmp34 = nLen * tmp32;
+	real tmp36 = tmp34 * tmp5;
+	real tmp37 = tmp36 * tmp6;
+	real tmp51 = nLen * tmp5;
+	real tmp53 = tmp51 * tmp6;
+	Y[0] = invDenom * (X[3] + -X[0]) * h * C;
+	for (int i = 0; i < 3; ++i)
+		Y[1 + i] = invDenom * (h * (nU[i] * tmp10 + nU[i] * tmp16 + -e[1 + i] * tmp22 + n2U[i] * tmp27
+			+ e[1 + i] * tmp37 + n3U[i] * tmp31)) / tmp53;
+}
+
+void oracle_sw_flux_from_cons(const oracle_params *p, const double *U, int s, double *F) {
+	/* shallow-water.cl:186-206 (extraTermInFlux = false) */
+	real m_n = U[1 + s];
+	real v_n = m_n / U[0];
+	real hP = .5 * p->gravity * U[0] * (U[0]);
+	F[0] = m_n;
+	for (int i = 0; i < 3; ++i) F[1 + i] = U[1 + i] * v_n + nl(1, 1 + i, s) * hP;
+	F[4] = 0.;
+}
+
+void oracle_sw_waves(const oracle_params *p, const double *e, int s, double *l) {
+	/* eigenWaveCodePrefix / eigenWaveCode (shallow-water.lua:247-275) */
+	real CSq = p->gravity * e[SW_H];
+	real C = sqrt(CSq);
+	real C_nLen = C * 1.;
+	real v_n = e[1 + s];
+	if (C <= 0.) { C_nLen = 0.; v_n = 0.; }
+	l[0] = v_n - C_nLen;
+	l[1] = v_n;
+	l[2] = v_n;
+	l[3] = v_n + C_nLen;
+}
+
+double oracle_sw_dt_cell(const oracle_params *p, const double *U, int dim, const double *dx) {
+	/* calcDT.cl:38-73 with consWaveCodeMinMaxAllSides (shallow-water.lua:318-354) */
+	real CSq = p->gravity * U[0];
+	real C = sqrt(CSq);
+	real W[5];
+	oracle_sw_prim_from_cons(p, U, W);
+	real dt = INFINITY;
+	for (int s = 0; s < dim; ++s) {
+		if (dx[s] > 0) {
+			real C_nLen = C * 1.;
+			real v_n = W[1 + s];
+			if (C <= 0.) { C_nLen = 0.; v_n = 0.; }
+			real lmin = v_n - C_nLen, lmax = v_n + C_nLen;
+			real absLambdaMax = fmax(fabs(lmin), fabs(lmax));
+			absLambdaMax = fmax(1e-9, absLambdaMax);
+			dt = fmin(dt, dx[s] / absLambdaMax);
+		}
+	}
+	return dt;
+}
+
 /* ------------------------------------------------------------------- Roe */
 
 static void eqn_eigen(const oracle_params *p, int eqn, const real *UL, const real *UR, real *e) {
 	if (eqn == HYDRO_EQN_EULER) oracle_euler_eigen_for_interface(p, UL, UR, e);
+	else if (eqn == HYDRO_EQN_SHALLOW_WATER) oracle_sw_eigen_for_interface(p, UL, UR, e);
 	else oracle_glm_eigen_for_interface(p, UL, UR, e);
 }
 static void eqn_left(const oracle_params *p, int eqn, const real *e, const real *X, int s, real *Y) {
 	if (eqn == HYDRO_EQN_EULER) oracle_euler_left(p, e, X, s, Y);
+	else if (eqn == HYDRO_EQN_SHALLOW_WATER) oracle_sw_left(p, e, X, s, Y);
 	else oracle_glm_left(p, e, X, s, Y);
 }
 static void eqn_right(const oracle_params *p, int eqn, const real *e, const real *X, int s, real *Y) {
 	if (eqn == HYDRO_EQN_EULER) oracle_euler_right(p, e, X, s, Y);
+	else if (eqn == HYDRO_EQN_SHALLOW_WATER) oracle_sw_right(p, e, X, s, Y);
 	else oracle_glm_right(p, e, X, s, Y);
 }
 static void eqn_flux(const oracle_params *p, int eqn, const real *U, int s, real *F) {
 	if (eqn == HYDRO_EQN_EULER) oracle_euler_flux_from_cons(p, U, s, F);
+	else if (eqn == HYDRO_EQN_SHALLOW_WATER) oracle_sw_flux_from_cons(p, U, s, F);
 	else oracle_glm_flux_from_cons(p, U, s, F);
 }
 static void eqn_waves(const oracle_params *p, int eqn, const real *e, int s, real *l) {
 	if (eqn == HYDRO_EQN_EULER) oracle_euler_waves(p, e, s, l);
+	else if (eqn == HYDRO_EQN_SHALLOW_WATER) oracle_sw_waves(p, e, s, l);
 	else oracle_glm_waves(p, e, s, l);
 }
-static int eqn_S(int eqn) { return eqn == HYDRO_EQN_EULER ? 6 : 12; }
-static int eqn_q(int eqn) { return eqn == HYDRO_EQN_EULER ? 5 : 8; }
+static int eqn_S(int eqn) { return eqn == HYDRO_EQN_EULER ? 6 : eqn == HYDRO_EQN_SHALLOW_WATER ? 5 : 12; }
+static int eqn_q(int eqn) { return eqn == HYDRO_EQN_EULER ? 5 : eqn == HYDRO_EQN_SHALLOW_WATER ? 4 : 8; }
 
 void oracle_roe_flux(const oracle_params *p, int eqn, int flux_limiter, double dt_dx,
 	const double *UL, const double *UR, const double *const *lim, int s, double *F)
@@ -1118,7 +1262,9 @@ static real *zalloc(size_t n) {
 
 oracle_solver *oracle_create(const hydro_cfg *cfg) {
 	if (cfg->dim < 1 || cfg->dim > 3) { set_err("dim must be 1..3"); return NULL; }
-	if (cfg->eqn != HYDRO_EQN_EULER && cfg->eqn != HYDRO_EQN_GLM_MAXWELL) { set_err("bad eqn"); return NULL; }
+	if (cfg->eqn != HYDRO_EQN_EULER && cfg->eqn != HYDRO_EQN_GLM_MAXWELL && cfg->eqn != HYDRO_EQN_SHALLOW_WATER) {
+		set_err("bad eqn"); return NULL;
+	}
 	if (cfg->plm && cfg->flux_limiter != HYDRO_LIMITER_DONOR_CELL) {
 		set_err("are you sure you want to use flux and slope limiters at the same time?");
 		return NULL;
@@ -1137,6 +1283,7 @@ oracle_solver *oracle_create(const hydro_cfg *cfg) {
 	s->p.gamma = cfg->gamma; s->p.rho_min = cfg->rho_min; s->p.P_min = cfg->P_min;
 	s->p.div_phi = cfg->div_phi_wavespeed; s->p.div_psi = cfg->div_psi_wavespeed;
 	s->p.roe_fix = cfg->roe_entropy_fix; s->p.roe_fix_factor = cfg->roe_entropy_fix_factor;
+	s->p.gravity = cfg->gravity;
 	s->dim = cfg->dim;
 	s->S = eqn_S(cfg->eqn);
 	s->q = eqn_q(cfg->eqn);
@@ -1204,7 +1351,7 @@ static void bc_apply(oracle_solver *s, real *buf, int side, int minmax, long dst
 	case HYDRO_BC_MIRROR:
 		memcpy(d, buf + src * S, S * sizeof(real));
 		/* reflectVars.mirror (eqn.lua:366-370): vector component along side */
-		if (s->cfg.eqn == HYDRO_EQN_EULER) {
+		if (s->cfg.eqn == HYDRO_EQN_EULER || s->cfg.eqn == HYDRO_EQN_SHALLOW_WATER) {
 			d[1 + side] = -1 * d[1 + side];
 		} else {
 			d[side] = -1 * d[side];
@@ -1352,8 +1499,8 @@ double oracle_calc_dt(oracle_solver *s) {
 		for (int d = 0; d < s->dim; ++d) if (iv[d] < NG || iv[d] >= s->gs[d] - NG) oob = 1;
 		if (!oob) {
 			const real *U = s->U + index * S;
-			dtBuf[index] = s->cfg.eqn == HYDRO_EQN_EULER
-				? oracle_euler_dt_cell(&s->p, U, s->dim, s->dx)
+			dtBuf[index] = s->cfg.eqn == HYDRO_EQN_EULER ? oracle_euler_dt_cell(&s->p, U, s->dim, s->dx)
+				: s->cfg.eqn == HYDRO_EQN_SHALLOW_WATER ? oracle_sw_dt_cell(&s->p, U, s->dim, s->dx)
 				: glm_dt_cell(&s->p, U, s->dim, s->dx);
 		}
 	});
@@ -1493,6 +1640,28 @@ static void glm_add_source(oracle_solver *s, real *deriv, const real *Ubuf) {
 	});
 }
 
+static void sw_add_source(oracle_solver *s, real *deriv, const real *Ubuf) {
+	/* shallow-water.cl:366-428, the enabled block: the second-order central
+	 * difference of depth (makePartial1 with numGhost = 2: coefficient 1,
+	 * denominator 2 dx; makepartial.lua:13-20, 85-110, eqn.lua:1345-1350),
+	 * divided by the basis lengths (1) and raised (identity), times g h,
+	 * added to the momentum derivative */
+	int S = s->S, dim = s->dim;
+	FOR_CELLS(s, NG, NG, {
+		real *dv = deriv + index * S;
+		const real *U = Ubuf + index * S;
+		real partial_depth_l[3] = {0., 0., 0.};
+		for (int j = 0; j < dim; ++j) {
+			const real *Up = U + s->step[j] * S, *Um = U - s->step[j] * S;
+			partial_depth_l[j] = (0. + (Up[4] - Um[4]) * 1.) * (1. / (s->dx[j] * 2.));
+		}
+		real e_depth_u[3];
+		for (int j = 0; j < 3; ++j) e_depth_u[j] = partial_depth_l[j] / 1.;
+		real gh = s->p.gravity * U[0];
+		for (int j = 0; j < 3; ++j) dv[1 + j] = dv[1 + j] + e_depth_u[j] * gh;
+	});
+}
+
 void oracle_calc_deriv(oracle_solver *s, double dt, double *deriv) {
 	/* FiniteVolumeSolver:calcDeriv (fvsolver.lua:225-302) + addSource
 	 * (solverbase.lua:3208-3214); the derivative buffer is cleared first
@@ -1502,6 +1671,7 @@ void oracle_calc_deriv(oracle_solver *s, double dt, double *deriv) {
 	calc_flux(s, s->U, dt);
 	calc_deriv_from_flux(s, deriv);
 	if (s->cfg.eqn == HYDRO_EQN_GLM_MAXWELL) glm_add_source(s, deriv, s->U);
+	if (s->cfg.eqn == HYDRO_EQN_SHALLOW_WATER) sw_add_source(s, deriv, s->U);
 }
 
 static void mult_add(oracle_solver *s, real *a, const real *b, const real *c, real d) {

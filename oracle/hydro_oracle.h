@@ -25,6 +25,7 @@ typedef struct oracle_params {
 	double gamma, rho_min, P_min, div_phi, div_psi;
 	int roe_fix;                 /* Roe, Euler: Harten's entropy fix (roe.cl:83-106) */
 	double roe_fix_factor;
+	double gravity;              /* shallow water */
 } oracle_params;
 
 oracle_solver *oracle_create(const hydro_cfg *cfg);
@@ -71,6 +72,14 @@ void oracle_glm_left(const oracle_params *p, const double *eig, const double *X,
 void oracle_glm_right(const oracle_params *p, const double *eig, const double *X, int side, double *Y);
 void oracle_glm_flux_from_cons(const oracle_params *p, const double *U, int side, double *F);
 void oracle_glm_waves(const oracle_params *p, const double *eig, int side, double *lambda);
+/* shallow water (depth in cons_t: h, m.xyz, depth; eigen h, v.xyz, C, depth) */
+void oracle_sw_prim_from_cons(const oracle_params *p, const double *U, double *W);
+void oracle_sw_eigen_for_interface(const oracle_params *p, const double *UL, const double *UR, double *eig);
+void oracle_sw_left(const oracle_params *p, const double *eig, const double *X, int side, double *Y);
+void oracle_sw_right(const oracle_params *p, const double *eig, const double *X, int side, double *Y);
+void oracle_sw_flux_from_cons(const oracle_params *p, const double *U, int side, double *F);
+void oracle_sw_waves(const oracle_params *p, const double *eig, int side, double *lambda);
+double oracle_sw_dt_cell(const oracle_params *p, const double *U, int dim, const double *dx);
 /* HLL ('Davis direct bounded', hll.cl:13-73) and Rusanov (rusanov.cl:13-39)
  * fluxes of the Euler equations for one interface; F gets 6 reals. */
 void oracle_hll_flux(const oracle_params *p, const double *UL, const double *UR, int side, double *F);

@@ -27,7 +27,7 @@ class OracleParams(ctypes.Structure):
     _fields_ = [('gamma', ctypes.c_double), ('rho_min', ctypes.c_double),
                 ('P_min', ctypes.c_double), ('div_phi', ctypes.c_double),
                 ('div_psi', ctypes.c_double), ('roe_fix', ctypes.c_int),
-                ('roe_fix_factor', ctypes.c_double)]
+                ('roe_fix_factor', ctypes.c_double), ('gravity', ctypes.c_double)]
 
 
 def lib():
@@ -87,6 +87,14 @@ def lib():
         L.oracle_plm_eig_prim.argtypes = [pp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp]
         for f in ('oracle_hll_flux', 'oracle_rusanov_flux', 'oracle_hllc_flux', 'oracle_marquina_flux'):
             getattr(L, f).argtypes = [pp, _dp, _dp, ctypes.c_int, _dp]
+        L.oracle_sw_prim_from_cons.argtypes = [pp, _dp, _dp]
+        L.oracle_sw_eigen_for_interface.argtypes = [pp, _dp, _dp, _dp]
+        for f in ('oracle_sw_left', 'oracle_sw_right'):
+            getattr(L, f).argtypes = [pp, _dp, _dp, ctypes.c_int, _dp]
+        for f in ('oracle_sw_flux_from_cons', 'oracle_sw_waves'):
+            getattr(L, f).argtypes = [pp, _dp, ctypes.c_int, _dp]
+        L.oracle_sw_dt_cell.restype = ctypes.c_double
+        L.oracle_sw_dt_cell.argtypes = [pp, _dp, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -95,8 +103,9 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
-def params(gamma=1.4, rho_min=1e-7, P_min=1e-7, div_phi=1.0, div_psi=1.0, roe_fix=0, roe_fix_factor=.2):
-    return OracleParams(gamma, rho_min, P_min, div_phi, div_psi, roe_fix, roe_fix_factor)
+def params(gamma=1.4, rho_min=1e-7, P_min=1e-7, div_phi=1.0, div_psi=1.0, roe_fix=0, roe_fix_factor=.2,
+           gravity=1.0):
+    return OracleParams(gamma, rho_min, P_min, div_phi, div_psi, roe_fix, roe_fix_factor, gravity)
 
 
 class Oracle:

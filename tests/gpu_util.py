@@ -36,3 +36,41 @@ def pair(init, **kw):
     o.boundary()
     o.constrain_U()
     return s, o
+
+
+def oracle_sensitivity(o_cfg, U0, dim, steps, rel, nq, trials=3, seed=1):
+    """How far the FP64 oracle's own result after `steps` steps moves when
+    its input U0 (ghosted, float64) is perturbed once by uniform relative
+    noise of size `rel` (one rounding of the stored precision): per
+    conserved variable, the largest relative L1 (absolute where the
+    variable's reference total is 0, as conftest.rel_l1 takes it) over
+    `trials` perturbations. Where the
+    reference's arithmetic has jumps (calc_Cs is 0 at P <= PMin and
+    sqrt(gamma P / rho) above it; 'ppm' takes phi(0) where a difference is
+    exactly 0), a one-rounding change of the input moves the result by more
+    than a rounding, and a result computed in another arithmetic can differ
+    from the oracle by as much."""
+    import pyoracle
+    from conftest import rel_l1
+
+    def run(U):
+        o = pyoracle.Oracle(o_cfg)
+        o.set_U(U)
+        o.boundary()
+        o.constrain_U()
+        o.update(steps)
+        return interior(dim, o.U)[..., :nq].copy()
+    ref = run(U0)
+    rng = np.random.default_rng(seed)
+    worst = np.zeros(nq)
+    for _ in range(trials):
+        out = run(U0 * (1 + rel * rng.uniform(-1, 1, U0.shape)))
+        assert np.all(np.isfinite(out))
+        worst = np.maximum(worst, rel_l1(out, ref))
+    return worst
+
+
+def init_euler_ghosted(init, cfg, dtype):
+    """init(cfg) as the ghosted cons_t buffer, rounded to dtype, in float64"""
+    from hydro.init import euler as init_euler
+    return init_euler.to_ghosted(cfg, init(cfg)).astype(dtype).astype(np.float64)

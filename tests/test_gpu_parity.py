@@ -328,8 +328,14 @@ def test_kh_fp32_fast_vs_fp64_oracle():
 @pytest.mark.parametrize('plm,n', [('plm cons', (128, 96)), ('ppm', (90, 70)), (None, (80, 64))])
 def test_glm_maxwell_2d_fp32_vs_fp64_oracle(plm, n):
     """FP32 GLM-Maxwell (the serving mix's third kind: periodic, RK2-TVD,
-    sigma = 0) against the FP64 oracle from the same float inputs"""
-    from gpu_util import pair
+    sigma = 0) against the FP64 oracle from the same float inputs: <= 1e-5.
+    'ppm' (not in the serving mix) is held to ten times the oracle's own
+    response to one float rounding of its input instead: its phi(dUL / dUR)
+    takes phi(0) where dUR is exactly 0 and phi(+-inf) beside it, so on the
+    pulses' flat background a rounding of the input moves the oracle's FP64
+    result by 6-7e-5 after 40 steps ('plm cons': 2.5e-7), and a float run
+    differs from it by as much."""
+    from gpu_util import init_euler_ghosted, oracle_sensitivity, pair
     kw = dict(dim=2, gridSize=n, eqn='glm_maxwell', usePLM=plm, slopeLimiter='minmod',
               integrator='Runge-Kutta 2, TVD', cfl=.4, boundary='periodic', real='float', fastMath=True)
     s, o = pair(_glm(14, 0.), **kw)
@@ -338,8 +344,14 @@ def test_glm_maxwell_2d_fp32_vs_fp64_oracle(plm, n):
     got = _interior(2, s.UBufObj.toCPU()).astype(np.float64)
     ref = _interior(2, o.U)
     err = rel_l1(got[..., :8], ref[..., :8])
-    print('glm fp32', plm, err)
-    assert np.all(err <= TOL32), err
+    sens = np.zeros(8)
+    if plm == 'ppm':
+        import pyoracle
+        U0 = init_euler_ghosted(_glm(14, 0.), s.cfg, np.float32)
+        sens = oracle_sensitivity(pyoracle.make_cfg(**_oracle_kw(kw)), U0, 2, 40, 2. ** -24, 8)
+    bar = np.maximum(TOL32, 10 * sens)
+    print('glm fp32', plm, err, 'oracle sensitivity', sens, 'bar', bar)
+    assert np.all(err <= bar), (err, bar)
     assert abs(s.t - o.t) <= TOL32 * o.t
 
 

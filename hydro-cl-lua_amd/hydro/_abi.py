@@ -17,7 +17,7 @@ EXPORTS = [
     'hydro_integrator_table',
     'hydro_device_count', 'hydro_create', 'hydro_destroy', 'hydro_num_cells', 'hydro_upload_U',
     'hydro_download_U', 'hydro_constrain_U', 'hydro_boundary', 'hydro_calc_dt', 'hydro_step',
-    'hydro_update', 'hydro_get_time', 'hydro_set_time', 'hydro_conserved_totals',
+    'hydro_update', 'hydro_get_time', 'hydro_set_time', 'hydro_conserved_totals', 'hydro_conserved_totals_device',
     'hydro_var_stats', 'hydro_sync',
     'hydro_time_updates', 'hydro_device_bytes', 'hydro_set_launch', 'hydro_get_launch',
     'hydro_batch_create', 'hydro_batch_destroy', 'hydro_batch_size', 'hydro_batch_device_bytes',
@@ -83,6 +83,8 @@ def lib():
     L.hydro_get_time.argtypes = [vp, dp, dp]
     L.hydro_set_time.argtypes = [vp, ctypes.c_double]
     L.hydro_conserved_totals.argtypes = [vp, dp]
+    ll = ctypes.c_longlong
+    L.hydro_conserved_totals_device.argtypes = [vp, ip, ip, ll, ll, vp, vp]
     L.hydro_var_stats.argtypes = [vp, ip, dp]
     L.hydro_time_updates.argtypes = [vp, ip, ip, ctypes.POINTER(HydroTiming)]
     L.hydro_set_launch.argtypes = [vp, ip, ip]

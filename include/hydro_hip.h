@@ -213,6 +213,20 @@ int hydro_set_time(hydro_solver *h, double t);
  * out has numIntStates entries. */
 int hydro_conserved_totals(hydro_solver *h, double *out);
 
+/* The same totals of a state the caller holds on the device, with no device
+ * memory allocated in the call. No counterpart in the reference: this is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). U: num_fields planes of num_cells
+ * reals (precision HYDRO_REAL_*), plane_stride reals apart, the layout of the
+ * solver's interior state; out: num_fields doubles on the device; stream: a
+ * hipStream_t, NULL for the default stream. Asynchronous on that stream. The
+ * per-block partial sums go to a 192 KiB library-static device array (one
+ * per GPU, kept until the process ends): num_fields is at most
+ * HYDRO_MAX_STATES, and calls that overlap on one GPU race on that array.
+ * Same partition and order as hydro_conserved_totals, so the same bits. */
+int hydro_conserved_totals_device(const void *U, int precision, int num_fields, long long plane_stride,
+	long long num_cells, double *out, void *stream);
+
 /* Derived display variables of the Euler equations (euler.lua:240-260), for
  * hydro_var_stats: W = primFromCons(U) per cell, values in double. */
 enum {

@@ -45,7 +45,9 @@ struct DevScalars {
 	double dtmin[3][DT_LANES];
 	double t;
 	double dt;
-	double pad[6];
+	unsigned redoCount;       // entries of the redo list of the running stage
+	unsigned redoDone;        // waves of the redo kernel that have finished
+	double pad[5];
 };
 
 // Boundary description of one axis, resolved on the host.
@@ -82,6 +84,7 @@ struct GridDesc {
 	long itemBase;            // batch: first work item (wave strip) of this grid
 	long nItems;              // stripsX * stripsY
 	DevScalars *dev;          // dt slots and time of this grid
+	int2 *redo;               // redo list: (strip item, first row) per strip, at most one each
 	int stepsLeft;            // batch: steps still to run in the current tick
 	int stripsZ;              // 3D: plane chunks of rowsPerWave planes
 };
@@ -114,6 +117,7 @@ struct StageArgs {
 	int resetSlot;            // slot to clear to +inf, -1 none
 	int reduceSlot;           // slot to min-reduce the new state's dt into, -1 none
 	int advanceTime;          // t += dt, dt stored
+	int redoPass;             // this launch is the redo kernel of the stage
 	int stepInTick;           // batch: skip grids with stepsLeft <= stepInTick
 	// equation parameters in the stencil type and in double (for the FP64
 	// accumulation, constrainU and dt work of the FP32 form)
@@ -132,13 +136,14 @@ struct StageArgs {
 
 // ---------------------------------------------------------------- lane moves
 // Lane l receives lane l-1's value (wave_shr:1) / lane l+1's (wave_shl:1);
-// the lane without a source (0 / 63) receives 0. With zero fill, FP32 users
-// of a shifted value take the shift as a DPP operand (no move).
+// the lane without a source (0 / 63) receives 0 (bound_ctrl: no old value to
+// set up first). With zero fill, FP32 users of a shifted value take the shift
+// as a DPP operand (no move).
 __device__ __forceinline__ int dpp_from_left_i(int v) {
-	return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);
+	return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true);
 }
 __device__ __forceinline__ int dpp_from_right_i(int v) {
-	return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false);
+	return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true);
 }
 __device__ __forceinline__ double from_left(double v) {
 	int2 p = __builtin_bit_cast(int2, v);

@@ -326,15 +326,16 @@ struct EulerT {
 		//   c (vn drho - dm_n): y0,4 = (dp +- c (vn drho - dm_n)) / (2 c^2),
 		//   y1 = drho - dp / c^2, tangential y = dm_t - v_t drho,
 		//   R in the matching factored form,
-		// c and 1 / c from one reciprocal square root. Returns false, with F
-		// unwritten, when any lane of the wave is not regular: the caller then
-		// takes the general form, which has every branch.
+		// c and 1 / c from one reciprocal square root. Computed whatever the
+		// states; returns whether this lane's interface is regular (F holds
+		// the result only then: otherwise the caller takes the general form,
+		// which has every branch).
 		template <int S>
-		__device__ static __forceinline__ bool roe_regular(const Par &p, const R *UL, const R *UR, R *F) {
+		__device__ static __forceinline__ bool roe_regular_lane(const Par &p, const R *UL, const R *UR, R *F) {
 			static_assert(FAST, "the collected Roe flux belongs to the FAST policy");
 			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
 			const R rhoFloor = fmax(R(1e-5), p.rhoMin);
-			if (!__all(UL[0] >= rhoFloor && UR[0] >= rhoFloor)) return false;   // NaN: not regular
+			const bool okRho = UL[0] >= rhoFloor && UR[0] >= rhoFloor;   // NaN: not regular
 			const R iL = M::rcp(UL[0]), iR = M::rcp(UR[0]);
 			R vL[3], vR[3];
 #pragma unroll
@@ -353,7 +354,7 @@ struct EulerT {
 			const R H = HR + wl * (HL - HR);
 			const R hq = R(.5) * lenSq3(v);
 			const R h = H - hq;
-			if (!__all(h >= R(1e-5))) return false;
+			const bool ok = okRho && h >= R(1e-5);
 			// c = sqrt(g1 h) and 1 / c: two Goldschmidt steps from v_rsq
 			const R c2 = p.g1 * h;
 			R s, hh;
@@ -403,7 +404,7 @@ struct EulerT {
 				}
 			}
 			F[4] = R(.5) * ((EPL * vnL + EPR * vnR) - z4);
-			return true;
+			return ok;
 		}
 
 		// eigenWaveCode (euler.lua:309-325)

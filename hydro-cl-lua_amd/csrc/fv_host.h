@@ -172,7 +172,28 @@ struct Grid {
 	int rowsPerWave;
 	int nvel;           // Euler: velocity components that are not identically zero
 	std::vector<double> ePot;   // Euler ePot of the last upload (dynamics-neutral)
+	void *redo;         // redo list of the stage kernels (see redo_kernels)
 };
+
+// Stage kernels (plain and batched) that leave strips with an irregular Roe
+// interface to a redo kernel (fv_stage.h, ROE_FLAGGED): FAST, Euler, Roe
+// without the entropy fix and without a flux limiter, 1D and 2D
+static inline bool redo_kernels(const hydro_cfg &c) {
+	const bool lim = c.plm == HYDRO_PLM_NONE && c.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
+	return c.fast_math && c.eqn == HYDRO_EQN_EULER && c.flux == HYDRO_FLUX_ROE && !c.roe_entropy_fix && !lim
+		&& c.dim <= 2;
+}
+
+// Blocks of the redo kernel, whose waves stride over the redo lists.
+static constexpr int REDO_BLOCKS = 64;
+
+// capacity of the redo list: one entry per strip at one row per strip, the
+// most strips any launch shape makes
+static inline size_t redo_capacity(const Grid &g) {
+	if (!redo_kernels(g.cfg)) return 0;
+	const long so = strip_out(strip_edges(g.dim, eqn_has_source(g.cfg.eqn)));
+	return (size_t)((g.n[0] + so - 1) / so) * (size_t)(g.dim >= 2 ? g.n[1] : 1);
+}
 
 static size_t elem_size(const hydro_cfg &c) { return c.precision == HYDRO_REAL_F32 ? 4 : 8; }
 
@@ -310,6 +331,7 @@ static size_t grid_bytes(const Grid &g) {
 	}
 	if (o == 1) b += state;
 	if (g.NSTAT) b += align256(plane * g.NSTAT);
+	b += align256(redo_capacity(g) * sizeof(int2));
 	return b;
 }
 
@@ -331,7 +353,11 @@ static void grid_bind(Grid &g, char *base) {
 		g.ping = p;
 		p += state;
 	}
-	if (g.NSTAT) g.stat = p;
+	if (g.NSTAT) {
+		g.stat = p;
+		p += align256(plane * g.NSTAT);
+	}
+	g.redo = redo_capacity(g) ? p : nullptr;
 }
 
 // zero every state buffer, +inf every dt slot (asynchronous on s)
@@ -354,6 +380,7 @@ static void grid_desc(const Grid &g, GridDesc<R> &d) {
 	}
 	d.ping = (R *)g.ping;
 	d.stat = (const R *)g.stat;
+	d.redo = (int2 *)g.redo;
 	for (int i = 0; i < 3; ++i) d.n[i] = g.n[i];
 	d.ps = g.ps;
 	d.step1 = g.n[0];
@@ -448,6 +475,7 @@ static void stage_args(const Grid &g, int s, StageArgs<R> &a) {
 	a.dtSlot = -1;
 	a.advanceTime = 0;
 	a.xcdRemap = 1;
+	a.redoPass = 0;
 }
 
 static StageKey stage_key(const Grid &g) {
@@ -691,6 +719,13 @@ static int launch_batched_stage(const Grid &rep, int nvel, int s, int cur, const
 	k.nvel = nvel;
 	hipError_t e = launch_stage<R>(k, a, dim3(0), dim3(64 * wavesPerBlock), stream);
 	if (e != hipSuccess) return fail("batched stage launch: %s", hipGetErrorString(e));
+	if (redo_kernels(rep.cfg)) {
+		// the strips the stage kernel left to the general Roe form (usually none)
+		a.redoPass = 1;
+		a.nBlocks = REDO_BLOCKS;
+		e = launch_stage<R>(k, a, dim3(REDO_BLOCKS), dim3(256), stream);
+		if (e != hipSuccess) return fail("batched redo kernel launch: %s", hipGetErrorString(e));
+	}
 	return 0;
 }
 

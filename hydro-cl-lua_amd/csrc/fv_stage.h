@@ -440,6 +440,26 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 	for (int j = 0; j < E::Q; ++j) F[j] = Y[j] + I.Favg[j];
 }
 
+// ------------------------------------------------ the Roe flux's two forms
+// How a FAST kernel of the Euler equations with the Roe flux and no flux
+// limiter computes an interface: ROE_INLINE takes, lane by lane, the collected
+// form where the interface is regular and the general form (every branch of
+// eigen_forInterface and the transforms) where it is not; ROE_FLAGGED takes
+// the collected form only and reports per lane whether the interface is
+// regular, so that the stage kernel leaves a strip from its first row with an
+// irregular interface on to the redo kernel (stage_kernel_redo), which runs
+// ROE_INLINE. Either way each interface's form depends on its own states only,
+// so a grid advances to the same bits whatever its strips' heights and
+// whether it runs alone or batched. Keeping the general form out of the stage
+// kernel keeps its registers at the collected form's (about 125 VGPRs FP64: 4
+// waves per SIMD, against 186 with both forms).
+enum { ROE_GENERAL = 0, ROE_INLINE = 1, ROE_FLAGGED = 2 };
+
+template <class E>
+__host__ __device__ constexpr bool roe_collected(bool lim) {
+	return E::FASTM && E::FLUXID == HYDRO_FLUX_ROE && E::Q == 5 && E::NW == 5 && !lim;
+}
+
 // ------------------------------------------------- one interface, no limiter
 // The interface flux of the equation's flux plugin without a flux limiter:
 // Roe (roe.cl:17-163), HLL with the 'Davis direct bounded' estimates
@@ -448,14 +468,18 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 // HLLC (euler-hllc.cl:13-240, hllcMethod 2, HLL's estimates), or Marquina
 // (marquina.cl:5-93). The cases of HLL and HLLC are selects over all
 // candidate fluxes, in the reference's expression order each.
-template <typename R, class E, int S>
+template <typename R, class E, int S, int RM = ROE_INLINE>
 __device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell<R, E> &UL, const Cell<R, E> &UR,
-	R dt_dx, int lim, R *F)
+	R dt_dx, int lim, R *F, bool *ok = nullptr)
 {
+	if (ok) *ok = true;
 	if constexpr (E::FLUXID == HYDRO_FLUX_ROE || E::FLUXID == FLUX_ROE_ENTROPY_FIX) {
-		if constexpr (E::FASTM && E::FLUXID == HYDRO_FLUX_ROE && E::Q == 5 && E::NW == 5) {
-			// the wave's interfaces are all regular: the collected form
-			if (E::template roe_regular<S>(p, UL.v, UR.v, F)) return;
+		if constexpr (roe_collected<E>(false) && RM == ROE_FLAGGED) {
+			*ok = E::template roe_regular_lane<S>(p, UL.v, UR.v, F);
+			return;
+		} else if constexpr (roe_collected<E>(false) && RM == ROE_INLINE) {
+			// a regular interface: the collected form
+			if (E::template roe_regular_lane<S>(p, UL.v, UR.v, F)) return;
 		}
 		Iface<R, E> I;
 		iface_prep<R, E, S>(p, UL, UR, I);
@@ -589,21 +613,23 @@ struct XEdge {
 	R fa[E::Q][S];
 	R u[U ? Cell<R, E>::NV : 1][S];
 	R dw[LIM ? E::NW : 1][S];
+	int ok[S];                // the interface is regular (ROE_FLAGGED kernels)
 };
 
 // x flux (area-weighted) and dW of the interface xb - 1/2 from the cells
 // xb-2 .. xb+1 of one row: the same arithmetic as the lanes' own interfaces
-template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM>
+template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM, int RM = ROE_INLINE>
 __device__ __forceinline__ void edge_flux(const typename E::Par &par, int lim, int slim, const Cell<R, E> *c, R dt_dx0,
-	bool fluxOn, R aov, R *Fa, R *dW)
+	bool fluxOn, R aov, R *Fa, R *dW, bool &ok)
 {
 	constexpr int Q = E::Q;
 	R F[Q];
+	ok = true;
 	if constexpr (PLM) {
 		Cell<R, E> L1, R1, L2, R2;
 		plm_cons<R, E, SLIM, 0>(par, slim, c[0], c[1], c[2], L1, R1, dt_dx0);
 		plm_cons<R, E, SLIM, 0>(par, slim, c[1], c[2], c[3], L2, R2, dt_dx0);
-		iface_solve<R, E, 0>(par, R1, L2, dt_dx0, lim, F);
+		iface_solve<R, E, 0, RM>(par, R1, L2, dt_dx0, lim, F, &ok);
 #pragma unroll
 		for (int j = 0; j < E::NW; ++j) dW[j] = R(0);   // kept by the flux limiter only
 	} else if constexpr (LIM) {
@@ -615,7 +641,7 @@ __device__ __forceinline__ void edge_flux(const typename E::Par &par, int lim, i
 #pragma unroll
 		for (int j = 0; j < E::NW; ++j) dW[j] = I.dW[j];
 	} else {
-		iface_solve<R, E, 0>(par, c[1], c[2], dt_dx0, lim, F);
+		iface_solve<R, E, 0, RM>(par, c[1], c[2], dt_dx0, lim, F, &ok);
 #pragma unroll
 		for (int j = 0; j < E::NW; ++j) dW[j] = R(0);
 	}
@@ -634,16 +660,18 @@ __device__ __forceinline__ void wave_lds_sync() {
 // wave's row j, lane NR + j the right one (lanes from 2 NR on repeat them and
 // store nothing); cb holds the lane's cells xb-2 .. xb+1 around its
 // interface xb - 1/2.
-template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM, int NR>
+template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM, int NR, int RM = ROE_INLINE>
 __device__ __forceinline__ void edge_pass(const typename E::Par &par, int lim, int slim, const Cell<R, E> *cb, R dt_dx0,
 	bool fluxOn, R aov, int lane, XEdge<R, E, PLM, LIM, NR> &edge)
 {
 	constexpr int Q = E::Q;
 	R Fa[Q], dW[E::NW];
-	edge_flux<R, E, PLM, LIM, FLIM, SLIM>(par, lim, slim, cb, dt_dx0, fluxOn, aov, Fa, dW);
+	bool ok;
+	edge_flux<R, E, PLM, LIM, FLIM, SLIM, RM>(par, lim, slim, cb, dt_dx0, fluxOn, aov, Fa, dW, ok);
 	const bool right = (lane & NR) != 0;
 	wave_lds_sync();   // the previous item's reads (persistent waves) are done
 	if (lane < 2 * NR) {
+		edge.ok[lane] = ok ? 1 : 0;
 #pragma unroll
 		for (int j = 0; j < Q; ++j) edge.fa[j][lane] = Fa[j];
 		if constexpr (XEdge<R, E, PLM, LIM, NR>::U) {
@@ -663,13 +691,14 @@ __device__ __forceinline__ void edge_pass(const typename E::Par &par, int lim, i
 // edges (lanes 0 and 63) taken from the row's edge slot j (without, lanes 0-1
 // and 62-63 are a halo whose results are not used). Um / Up return the x
 // neighbours.
-template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM, int NR, bool EDGES = true>
+template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM, int NR, bool EDGES = true, int RM = ROE_INLINE>
 __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int slim, const Cell<R, E> &U, R dt_dx0,
 	bool fluxOn, R aov, int lane, int j, const XEdge<R, E, PLM, LIM, NR> &edge, Cell<R, E> &Um, Cell<R, E> &Up,
-	R *divx)
+	R *divx, bool &okx)
 {
 	using C = Cell<R, E>;
 	constexpr int Q = E::Q;
+	okx = true;
 	const int slot = (lane == WAVE - 1 ? NR : 0) + j;
 	if constexpr (EDGES && XEdge<R, E, PLM, LIM, NR>::U) {
 		// lane 0 reads the left edge slot, lane 63 the right one: the shifts'
@@ -701,7 +730,7 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 #pragma unroll
 			for (int f = 0; f < C::NV; ++f) RxL.v[f] = lane == 0 ? Rx.v[f] : RxL.v[f];
 		}
-		iface_solve<R, E, 0>(par, RxL, Lx, dt_dx0, lim, Fx);
+		iface_solve<R, E, 0, RM>(par, RxL, Lx, dt_dx0, lim, Fx, &okx);
 	} else if constexpr (LIM) {
 		Iface<R, E> I;
 		iface_prep<R, E, 0>(par, Um, U, I);
@@ -721,7 +750,12 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 		}
 		iface_flux<R, E, 0, true, FLIM>(par, I, dWL, dWR, dt_dx0, lim, Fx);
 	} else {
-		iface_solve<R, E, 0>(par, Um, U, dt_dx0, lim, Fx);
+		iface_solve<R, E, 0, RM>(par, Um, U, dt_dx0, lim, Fx, &okx);
+	}
+	if constexpr (RM == ROE_FLAGGED && EDGES) {
+		// lane 0's interface is the left edge one; lane 63 also needs the right edge
+		const bool eok = edge.ok[slot] != 0;
+		okx = lane == 0 ? eok : (lane == WAVE - 1 ? okx && eok : okx);
 	}
 #pragma unroll
 	for (int k = 0; k < Q; ++k) {
@@ -742,9 +776,10 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 // Advances the strip (sx, sy) of grid g by one stage: 64 output columns (one
 // per lane; 1D: 60) by rowsPerWave (<= 32) rows. Returns the strip's dt minimum of the new
 // state when the stage reduces dt.
-template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
+template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM,
+	int RM = ROE_INLINE>
 __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridDesc<R> &g, int sx, int sy, double dt,
-	int lane, XEdge<R, EQ<R>, PLM, LIM, EDGE_ROWS> &edge)
+	int lane, XEdge<R, EQ<R>, PLM, LIM, EDGE_ROWS> &edge, int yStart = -1)
 {
 	using E = EQ<R>;
 	using C = Cell<R, E>;
@@ -768,8 +803,9 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	const R rdt = R(dt);
 	const R dt_dx0 = rdt / g.dx[0];
 	const R dt_dx1 = DIM >= 2 ? rdt / g.dx[1] : R(0);
-	const int y0 = sy * g.rowsPerWave;
-	const int y1 = min(y0 + g.rowsPerWave, ny);
+	// the strip's rows; the redo kernel starts at the first row it takes over
+	const int y0 = yStart >= 0 ? yStart : sy * g.rowsPerWave;
+	const int y1 = min(sy * g.rowsPerWave + g.rowsPerWave, ny);
 
 	// ---- the strip's edge interfaces for rows y0 .. y1-1: lane j (< 32) the
 	// left edge of row y0 + j, lane 32 + j the right edge
@@ -794,8 +830,8 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 #pragma unroll
 			for (int k = 0; k < 4; ++k) load_cell<R, E>(g, src, mxb[k], myb, cb[k]);
 		}
-		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS>(par, lim, slim, cb, dt_dx0, flux_on<E>(g, 0), flux_aov<E>(g, 0),
-			lane, edge);
+		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS, RM>(par, lim, slim, cb, dt_dx0, flux_on<E>(g, 0),
+			flux_aov<E>(g, 0), lane, edge);
 	}
 
 	// y pipeline state
@@ -804,6 +840,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	Iface<R, E> I1;         // LIM: interface r+1/2
 	R dW0[E::NW];           // LIM: dW of interface r-1/2
 	R FaYprev[Q];
+	bool okYprev = true;    // ROE_FLAGGED: interface r-1/2 is regular
 #pragma unroll
 	for (int f = 0; f < C::NV; ++f) { Wm.v[f] = W0.v[f] = W1.v[f] = W2.v[f] = Ry0.v[f] = R(0); }
 #pragma unroll
@@ -837,6 +874,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	for (int r = rBegin; r < rEnd; ++r) {
 		// ---- y direction: interface r+1/2, dyterm of row r
 		R dyterm[Q];
+		bool okY = true;
 		if constexpr (DIM >= 2) {
 			R F[Q];
 			// the first PLM warm-up iteration's interface is never used
@@ -847,7 +885,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 				C L1, R1;
 				plm_cons<R, E, SLIM, 1>(par, slim, W0, W1, W2, L1, R1, dt_dx1);
 				if (wantFlux) {
-					iface_solve<R, E, 1>(par, Ry0, L1, dt_dx1, lim, F);
+					iface_solve<R, E, 1, RM>(par, Ry0, L1, dt_dx1, lim, F, &okY);
 				} else {
 #pragma unroll
 					for (int j = 0; j < Q; ++j) F[j] = R(0);
@@ -865,7 +903,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			} else {
 				// W0 = U(r): interface r+1/2
 				loadRow(r + 1, W1);
-				iface_solve<R, E, 1>(par, W0, W1, dt_dx1, lim, F);
+				iface_solve<R, E, 1, RM>(par, W0, W1, dt_dx1, lim, F, &okY);
 			}
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
@@ -882,8 +920,20 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			else loadRow(0, U);
 			C Um, Up;
 			R deriv[Q];
-			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS, EDGES>(par, lim, slim, U, dt_dx0, flux_on<E>(g, 0),
-				flux_aov<E>(g, 0), lane, r - y0, edge, Um, Up, deriv);
+			bool okX;
+			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS, EDGES, RM>(par, lim, slim, U, dt_dx0, flux_on<E>(g, 0),
+				flux_aov<E>(g, 0), lane, r - y0, edge, Um, Up, deriv, okX);
+			if constexpr (RM == ROE_FLAGGED) {
+				// an irregular interface of this row: the redo kernel takes the
+				// strip from this row on; nothing of it is stored here
+				if (!__all(okYprev && okY && okX)) {
+					if (lane == 0) {
+						const unsigned k = atomicAdd(&g.dev->redoCount, 1u);
+						g.redo[k] = make_int2(sy * g.stripsX + sx, r);
+					}
+					break;
+				}
+			}
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
 				if constexpr (DIM >= 2) deriv[j] = deriv[j] - dyterm[j];
@@ -954,6 +1004,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 		}
 
 		// ---- advance the row window
+		okYprev = okY;
 		if constexpr (DIM >= 2) {
 			if constexpr (E::SOURCE) Wm = W0;
 			if constexpr (PLM || LIM) { W0 = W1; W1 = W2; }
@@ -992,10 +1043,14 @@ __device__ __forceinline__ void stage_reduce(const StageArgs<R> &a, const GridDe
 }
 
 // ---------------------------------------------------------- the kernels
-// Plain form: one grid, one wave per strip.
+// Plain form: one grid, one wave per strip. The FAST kernels of the Euler
+// equations with the Roe flux and no flux limiter take the collected Roe form
+// only (ROE_FLAGGED) and leave any strip with an irregular interface, from
+// that row on, to stage_kernel_redo, launched after them on the same stream.
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
 __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 {
+	constexpr int RM = roe_collected<EQ<R>>(LIM) ? ROE_FLAGGED : ROE_INLINE;
 	const int lane = threadIdx.x & 63;
 	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
 	const int waveInBlock = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1007,8 +1062,54 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 	const int sx = (int)(item % g.stripsX);
 	const int sy = (int)(item / g.stripsX);
 	if (item == 0) stage_first_strip(a, g, dt, lane);
-	double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM>(a, g, sx, sy, dt, lane, edges[waveInBlock]);
+	double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, RM>(a, g, sx, sy, dt, lane, edges[waveInBlock]);
 	stage_reduce(a, g, dtLocal, item, lane);
+}
+
+// The strips a stage kernel left (from their first irregular row on), with
+// the Roe form chosen per interface (ROE_INLINE): the entries of every grid's redo list (one grid for
+// the plain form, a.grids for the batched one) are numbered one after the
+// other, and wave k takes entries k, k + W, k + 2W, ... (W = waves launched).
+// The rows it takes were not written by the stage kernel, so their inputs are
+// as the stage began; their dt joins the same slot. The last wave to finish
+// empties every list for the next stage (all waves have read the counts by
+// then). A stage with no irregular interface costs one read per grid.
+template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
+__global__ void __launch_bounds__(256) stage_kernel_redo(const StageArgs<R> a)
+{
+	const int lane = threadIdx.x & 63;
+	const int waveInBlock = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const long wave = (long)blockIdx.x * (blockDim.x >> 6) + waveInBlock;
+	const long nWaves = (long)gridDim.x * (blockDim.x >> 6);
+	__shared__ XEdge<R, EQ<R>, PLM, LIM, EDGE_ROWS> edges[4];
+	const bool batched = a.grids != nullptr;
+	const int nGrids = batched ? a.nGrids : 1;
+	long base = 0;     // entries of the grids before grid gi
+	for (int gi = 0; gi < nGrids; ++gi) {
+		const GridDesc<R> &g = batched ? a.grids[gi] : a.g;
+		const long n = (long)g.dev->redoCount;
+		if (n == 0) continue;
+		const double dt = stage_dt(a, g);
+		// this wave's first entry at or after base
+		for (long k = base + ((wave - base) % nWaves + nWaves) % nWaves; k < base + n; k += nWaves) {
+			const int2 e = g.redo[k - base];
+			const int sx = e.x % g.stripsX, sy = e.x / g.stripsX;
+			double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, ROE_INLINE>(a, g, sx, sy, dt, lane,
+				edges[waveInBlock], e.y);
+			stage_reduce(a, g, dtLocal, e.x, lane);
+		}
+		base += n;
+	}
+	DevScalars *d = (batched ? a.grids[0] : a.g).dev;
+	unsigned last = 0u;
+	if (lane == 0) {
+		__threadfence();
+		last = atomicAdd(&d->redoDone, 1u) == (unsigned)(nWaves - 1) ? 1u : 0u;
+	}
+	if (__shfl(last, 0)) {
+		for (int gi = lane; gi < nGrids; gi += 64) (batched ? a.grids[gi] : a.g).dev->redoCount = 0u;
+		if (lane == 0) d->redoDone = 0u;
+	}
 }
 
 // Dynamic LDS of the batched kernel: the strip-edge slices of its waves
@@ -1018,10 +1119,13 @@ extern __shared__ __align__(16) unsigned char batch_lds[];
 // Batched form: a persistent launch over every strip of every active grid of
 // the batch; wave k takes items k, k + W, k + 2W, ... (W = waves launched).
 // Items are numbered grid by grid (GridDesc::itemBase), so the owning grid of
-// an item is found by a wave-uniform binary search.
+// an item is found by a wave-uniform binary search. The Roe form is the plain
+// kernel's (ROE_FLAGGED where it applies, with the same redo kernel after it),
+// so a grid advances to the same bits batched or alone.
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
 __global__ void __launch_bounds__(256) stage_kernel_batch(const StageArgs<R> a)
 {
+	constexpr int RM = roe_collected<EQ<R>>(LIM) ? ROE_FLAGGED : ROE_INLINE;
 	const int lane = threadIdx.x & 63;
 	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
 	const int waveInBlock = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1045,7 +1149,7 @@ __global__ void __launch_bounds__(256) stage_kernel_batch(const StageArgs<R> a)
 		const int sx = (int)(local % g.stripsX);
 		const int sy = (int)(local / g.stripsX);
 		if (local == 0) stage_first_strip(a, g, dt, lane);
-		double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM>(a, g, sx, sy, dt, lane, edges[waveInBlock]);
+		double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, RM>(a, g, sx, sy, dt, lane, edges[waveInBlock]);
 		stage_reduce(a, g, dtLocal, local, lane);
 	}
 }

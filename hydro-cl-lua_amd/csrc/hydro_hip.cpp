@@ -161,9 +161,18 @@ static int solver_stage(hydro_solver *h, int s, int useSlot, double hostDt, int 
 	a.advanceTime = advance;
 	a.xcdRemap = h->xcdRemap;
 	a.nBlocks = (int)(g.dim == 3 ? a.g.nItems : (a.g.nItems + 3) / 4);
+	const bool redo = redo_kernels(g.cfg);
 	if (ev0) HIPCHECK(hipEventRecord(ev0, h->stream));
-	hipError_t e = launch_stage<R>(stage_key(g), a, dim3(a.nBlocks), dim3(256), h->stream);
+	const StageKey k = stage_key(g);
+	hipError_t e = launch_stage<R>(k, a, dim3(a.nBlocks), dim3(256), h->stream);
 	if (e != hipSuccess) return fail("stage kernel launch: %s", hipGetErrorString(e));
+	if (redo) {
+		// the strips the stage kernel left to the general Roe form (usually none)
+		a.redoPass = 1;
+		a.nBlocks = REDO_BLOCKS;
+		e = launch_stage<R>(k, a, dim3(REDO_BLOCKS), dim3(256), h->stream);
+		if (e != hipSuccess) return fail("redo kernel launch: %s", hipGetErrorString(e));
+	}
 	if (ev1) HIPCHECK(hipEventRecord(ev1, h->stream));
 	return 0;
 }

@@ -209,39 +209,45 @@ static int pool_for(hydro_server *s, const Request &r) {
 	return (int)s->pools.size() - 1;
 }
 
+// Moves queued requests that fit the arena to their pools: state upload, then
+// their dt into the pool's current slot. A request that fails any step gets its
+// arena block back and stays queued, ahead of those not yet looked at and
+// behind those that did not fit (the queue order is kept).
 static int admit(hydro_server *s) {
 	std::deque<long long> still;
-	std::vector<long long> admitted;
 	while (!s->queued.empty()) {
 		long long id = s->queued.front();
-		s->queued.pop_front();
 		Request &r = s->reqs.at(id);
 		long off = s->arena.alloc(r.bytes);
 		if (off < 0) {
+			s->queued.pop_front();
 			still.push_back(id);
 			continue;
 		}
 		r.off = (size_t)off;
 		grid_bind(r.g, s->arena.base + r.off);
-		if (grid_clear(r.g, s->arena.base + r.off, s->upload)) return -1;
-		if (grid_upload(r.g, r.U0.data(), r.U0.size(), s->upload)) return -1;   // synchronizes
+		int p = -1;
+		bool ok = !grid_clear(r.g, s->arena.base + r.off, s->upload)
+			&& !grid_upload(r.g, r.U0.data(), r.U0.size(), s->upload)      // synchronizes
+			&& (p = pool_for(s, r)) >= 0;
+		if (ok) {
+			hipError_t e = launch_reset_slots(r.g.dev, 7, s->upload);
+			if (e != hipSuccess) ok = !fail("reset of the dt slots: %s", hipGetErrorString(e));
+		}
+		ok = ok && (r.cfg.use_fixed_dt || !grid_dt_kernel(r.g, s->pools[p].slot, s->upload));
+		if (!ok) {
+			s->arena.release(r.off, r.bytes);
+			for (auto it = still.rbegin(); it != still.rend(); ++it) s->queued.push_front(*it);
+			return -1;
+		}
+		s->queued.pop_front();
 		r.U0.clear();
 		r.U0.shrink_to_fit();
-		int p = pool_for(s, r);
-		if (p < 0) return -1;
 		r.pool = p;
 		r.state = 1;
-		admitted.push_back(id);
+		s->pools[p].resident.push_back(id);
 	}
 	s->queued.swap(still);
-	// dt of each new request into its pool's current slot
-	for (long long id : admitted) {
-		Request &r = s->reqs.at(id);
-		Pool &p = s->pools[r.pool];
-		HIPCHECK(launch_reset_slots(r.g.dev, 7, s->upload));
-		if (!r.cfg.use_fixed_dt && grid_dt_kernel(r.g, p.slot, s->upload)) return -1;
-		p.resident.push_back(id);
-	}
 	HIPCHECK(hipStreamSynchronize(s->upload));
 	return 0;
 }

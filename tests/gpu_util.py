@@ -38,7 +38,7 @@ def pair(init, **kw):
     return s, o
 
 
-def oracle_sensitivity(o_cfg, U0, dim, steps, rel, nq, trials=3, seed=1):
+def oracle_sensitivity(o_cfg, U0, dim, steps, rel, nq, trials=3, seed=1, with_t=False):
     """How far the FP64 oracle's own result after `steps` steps moves when
     its input U0 (ghosted, float64) is perturbed once by uniform relative
     noise of size `rel` (one rounding of the stored precision): per
@@ -49,7 +49,8 @@ def oracle_sensitivity(o_cfg, U0, dim, steps, rel, nq, trials=3, seed=1):
     sqrt(gamma P / rho) above it; 'ppm' takes phi(0) where a difference is
     exactly 0), a one-rounding change of the input moves the result by more
     than a rounding, and a result computed in another arithmetic can differ
-    from the oracle by as much."""
+    from the oracle by as much. with_t: also the largest relative change of
+    the simulated time (dt follows the wave speeds, calc_Cs among them)."""
     import pyoracle
     from conftest import rel_l1
 
@@ -59,15 +60,17 @@ def oracle_sensitivity(o_cfg, U0, dim, steps, rel, nq, trials=3, seed=1):
         o.boundary()
         o.constrain_U()
         o.update(steps)
-        return interior(dim, o.U)[..., :nq].copy()
-    ref = run(U0)
+        return interior(dim, o.U)[..., :nq].copy(), o.t
+    ref, t_ref = run(U0)
     rng = np.random.default_rng(seed)
     worst = np.zeros(nq)
+    worst_t = 0.
     for _ in range(trials):
-        out = run(U0 * (1 + rel * rng.uniform(-1, 1, U0.shape)))
+        out, t = run(U0 * (1 + rel * rng.uniform(-1, 1, U0.shape)))
         assert np.all(np.isfinite(out))
         worst = np.maximum(worst, rel_l1(out, ref))
-    return worst
+        worst_t = max(worst_t, abs(t - t_ref) / abs(t_ref))
+    return (worst, worst_t) if with_t else worst
 
 
 def init_euler_ghosted(init, cfg, dtype):

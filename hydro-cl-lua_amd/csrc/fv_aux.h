@@ -36,6 +36,22 @@ struct GridArgs {
 	if (threadIdx.x == 0) dev->dtmin[slot][0] = m;
 }
 
+// Whether any stored value of a state is not finite: the q integrated and the
+// nstat static planes over the interior cells. Sets *flag to 1 and never
+// clears it (the caller starts it at 0).
+template <typename R>
+__global__ void __launch_bounds__(256) nonfinite_kernel(const R *U, const R *stat, long ps, long ncell, int q,
+	int nstat, unsigned *flag)
+{
+	bool bad = false;
+	for (int f = 0; f < q + nstat; ++f) {
+		const R *p = f < q ? U + (long)f * ps : stat + (long)(f - q) * ps;
+		for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x)
+			bad = bad || !__builtin_isfinite(p[c]);
+	}
+	if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
 // calcDT (eqn.lua:1187-1224 + calcDT.cl:38-73) and reduceMin: per-cell dt in
 // double, wave min, one atomic min per wave into dev->dtmin[slot].
 template <typename R, template <typename> class EQ, int DIM>

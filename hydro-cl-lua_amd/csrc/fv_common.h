@@ -46,8 +46,10 @@ struct DevScalars {
 	double t;
 	double dt;
 	unsigned redoCount;       // entries of the redo list of the running stage
-	unsigned redoDone;        // waves of the redo kernel that have finished
-	double pad[5];
+	unsigned redoDone;        // blocks of the redo kernel that have finished
+	unsigned nonFinite;       // set by nonfinite_kernel
+	unsigned pad0;
+	double pad[4];
 };
 
 // Boundary description of one axis, resolved on the host.

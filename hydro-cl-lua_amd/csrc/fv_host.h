@@ -361,7 +361,7 @@ static void grid_bind(Grid &g, char *base) {
 }
 
 // zero every state buffer, +inf every dt slot (asynchronous on s)
-static int grid_clear(Grid &g, char *base, hipStream_t s) {
+[[maybe_unused]] static int grid_clear(Grid &g, char *base, hipStream_t s) {
 	HIPCHECK(hipMemsetAsync(base, 0, grid_bytes(g), s));
 	DevScalars z;
 	std::memset(&z, 0, sizeof z);
@@ -532,15 +532,18 @@ static size_t ghosted_cells(const Grid &g) {
 	return c;
 }
 
+// AoS ghosted cons_t -> the device layout (q planes and nstat static planes of
+// ps values each, interior cells dense, the rest zero), into host memory the
+// caller provides; sets g.nvel and g.ePot from the state
 template <typename R>
-static void pack_interior(Grid &g, const R *aos, std::vector<R> &planes, std::vector<R> &stat) {
+static void pack_interior(Grid &g, const R *aos, R *planes, R *stat) {
 	int gs[3], o[3];
 	for (int i = 0; i < 3; ++i) {
 		gs[i] = i < g.dim ? g.n[i] + 2 * NG : 1;
 		o[i] = i < g.dim ? NG : 0;
 	}
-	planes.assign((size_t)g.Q * g.ps, R(0));
-	stat.assign((size_t)g.NSTAT * g.ps, R(0));
+	std::fill(planes, planes + (size_t)g.Q * g.ps, R(0));
+	if (g.NSTAT) std::fill(stat, stat + (size_t)g.NSTAT * g.ps, R(0));
 	bool anyE = false;
 	if (g.cfg.eqn == HYDRO_EQN_EULER) g.ePot.assign(g.ncell, 0.);
 	for (int k = 0; k < g.n[2]; ++k)
@@ -570,7 +573,7 @@ static void pack_interior(Grid &g, const R *aos, std::vector<R> &planes, std::ve
 }
 
 template <typename R>
-static void unpack_with_ghosts(const Grid &g, const std::vector<R> &planes, const std::vector<R> &stat, R *aos) {
+static void unpack_with_ghosts(const Grid &g, const R *planes, const R *stat, R *aos) {
 	int gs[3];
 	for (int i = 0; i < 3; ++i) gs[i] = i < g.dim ? g.n[i] + 2 * NG : 1;
 	const hydro_cfg &c = g.cfg;
@@ -601,8 +604,8 @@ static void unpack_with_ghosts(const Grid &g, const std::vector<R> &planes, cons
 
 template <typename R>
 static int grid_upload_t(Grid &g, const void *U, hipStream_t s) {
-	std::vector<R> planes, stat;
-	pack_interior<R>(g, (const R *)U, planes, stat);
+	std::vector<R> planes((size_t)g.Q * g.ps), stat((size_t)g.NSTAT * g.ps);
+	pack_interior<R>(g, (const R *)U, planes.data(), stat.data());
 	HIPCHECK(hipMemcpyAsync(g.U[0], planes.data(), planes.size() * sizeof(R), hipMemcpyHostToDevice, s));
 	if (g.NSTAT) HIPCHECK(hipMemcpyAsync(g.stat, stat.data(), stat.size() * sizeof(R), hipMemcpyHostToDevice, s));
 	HIPCHECK(hipStreamSynchronize(s));
@@ -616,18 +619,18 @@ static int grid_download_t(const Grid &g, void *U, hipStream_t s) {
 	HIPCHECK(hipMemcpyAsync(planes.data(), g.U[0], planes.size() * sizeof(R), hipMemcpyDeviceToHost, s));
 	if (g.NSTAT) HIPCHECK(hipMemcpyAsync(stat.data(), g.stat, stat.size() * sizeof(R), hipMemcpyDeviceToHost, s));
 	HIPCHECK(hipStreamSynchronize(s));
-	unpack_with_ghosts<R>(g, planes, stat, (R *)U);
+	unpack_with_ghosts<R>(g, planes.data(), stat.data(), (R *)U);
 	return 0;
 }
 
-static int grid_upload(Grid &g, const void *U, size_t bytes, hipStream_t s) {
+[[maybe_unused]] static int grid_upload(Grid &g, const void *U, size_t bytes, hipStream_t s) {
 	if (!U) return fail("NULL argument");
 	size_t need = ghosted_cells(g) * g.S * g.elem;
 	if (bytes != need) return fail("upload size %zu != %zu (numCells * numStates * sizeof(real))", bytes, need);
 	return g.elem == 8 ? grid_upload_t<double>(g, U, s) : grid_upload_t<float>(g, U, s);
 }
 
-static int grid_download(const Grid &g, void *U, size_t bytes, hipStream_t s) {
+[[maybe_unused]] static int grid_download(const Grid &g, void *U, size_t bytes, hipStream_t s) {
 	if (!U) return fail("NULL argument");
 	size_t need = ghosted_cells(g) * g.S * g.elem;
 	if (bytes != need) return fail("download size %zu != %zu (numCells * numStates * sizeof(real))", bytes, need);
@@ -678,7 +681,7 @@ static int grid_dt_kernel(Grid &g, int slot, hipStream_t s) {
 	return 0;
 }
 
-static int grid_get_time(const Grid &g, double *t, double *dt, hipStream_t s) {
+[[maybe_unused]] static int grid_get_time(const Grid &g, double *t, double *dt, hipStream_t s) {
 	double v[2];
 	HIPCHECK(hipMemcpyAsync(v, &g.dev->t, sizeof v, hipMemcpyDeviceToHost, s));
 	HIPCHECK(hipStreamSynchronize(s));

@@ -52,6 +52,11 @@ hipError_t launch_dt(int eqn, int dim, int fast, const GridArgs<R> &g, hipStream
 template <typename R>
 hipError_t launch_constrain(int eqn, int fast, const GridArgs<R> &g, hipStream_t s);
 
+// *flag = 1 when any value of the q + nstat planes (interior cells) is not finite
+template <typename R>
+hipError_t launch_nonfinite(const R *U, const R *stat, long ps, long ncell, int q, int nstat, unsigned *flag,
+	hipStream_t s);
+
 template <typename R>
 hipError_t launch_totals(const R *U, long ps, long ncell, int q, KSum *partial, int nPartial, double *out, hipStream_t s);
 

@@ -756,6 +756,10 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 		// lane 0's interface is the left edge one; lane 63 also needs the right edge
 		const bool eok = edge.ok[slot] != 0;
 		okx = lane == 0 ? eok : (lane == WAVE - 1 ? okx && eok : okx);
+	} else if constexpr (RM == ROE_FLAGGED) {
+		// halo strip: only the interfaces of the output lanes NG..63-NG count
+		// (the outer halo lanes' left states come from the zero fill)
+		okx = okx || lane < NG || lane > WAVE - NG;
 	}
 #pragma unroll
 	for (int k = 0; k < Q; ++k) {
@@ -1067,13 +1071,15 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 }
 
 // The strips a stage kernel left (from their first irregular row on), with
-// the Roe form chosen per interface (ROE_INLINE): the entries of every grid's redo list (one grid for
-// the plain form, a.grids for the batched one) are numbered one after the
-// other, and wave k takes entries k, k + W, k + 2W, ... (W = waves launched).
-// The rows it takes were not written by the stage kernel, so their inputs are
-// as the stage began; their dt joins the same slot. The last wave to finish
-// empties every list for the next stage (all waves have read the counts by
-// then). A stage with no irregular interface costs one read per grid.
+// the Roe form chosen per interface (ROE_INLINE): the entries of every grid's
+// redo list (one grid for the plain form, a.grids for the batched one) are
+// numbered one after the other, and wave k takes entries k, k + W, k + 2W, ...
+// (W = waves launched). The rows it takes were not written by the stage
+// kernel, so their inputs are as the stage began; their dt joins the same
+// slot. The lists' lengths are read 64 grids at a time, one per lane; when
+// every list is empty (the usual case) the kernel ends there, with nothing to
+// clear. Otherwise the last block to finish empties the lists for the next
+// stage (every wave has read the lengths by then); no wave depends on another.
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
 __global__ void __launch_bounds__(256) stage_kernel_redo(const StageArgs<R> a)
 {
@@ -1082,32 +1088,39 @@ __global__ void __launch_bounds__(256) stage_kernel_redo(const StageArgs<R> a)
 	const long wave = (long)blockIdx.x * (blockDim.x >> 6) + waveInBlock;
 	const long nWaves = (long)gridDim.x * (blockDim.x >> 6);
 	__shared__ XEdge<R, EQ<R>, PLM, LIM, EDGE_ROWS> edges[4];
+	__shared__ unsigned lastBlock;
 	const bool batched = a.grids != nullptr;
 	const int nGrids = batched ? a.nGrids : 1;
-	long base = 0;     // entries of the grids before grid gi
-	for (int gi = 0; gi < nGrids; ++gi) {
-		const GridDesc<R> &g = batched ? a.grids[gi] : a.g;
-		const long n = (long)g.dev->redoCount;
-		if (n == 0) continue;
-		const double dt = stage_dt(a, g);
-		// this wave's first entry at or after base
-		for (long k = base + ((wave - base) % nWaves + nWaves) % nWaves; k < base + n; k += nWaves) {
-			const int2 e = g.redo[k - base];
-			const int sx = e.x % g.stripsX, sy = e.x / g.stripsX;
-			double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, ROE_INLINE>(a, g, sx, sy, dt, lane,
-				edges[waveInBlock], e.y);
-			stage_reduce(a, g, dtLocal, e.x, lane);
+	long base = 0;     // entries of the grids before the current one
+	for (int g0 = 0; g0 < nGrids; g0 += 64) {
+		const int gi = g0 + lane;
+		unsigned c = 0u;
+		if (gi < nGrids) c = (batched ? a.grids[gi].dev : a.g.dev)->redoCount;
+		unsigned long long pending = __ballot(c != 0u);
+		while (pending) {
+			const int j = __ffsll((long long)pending) - 1;
+			pending &= pending - 1;
+			const long n = (long)__shfl(c, j);
+			const GridDesc<R> &g = batched ? a.grids[g0 + j] : a.g;
+			const double dt = stage_dt(a, g);
+			// this wave's first entry at or after base
+			for (long k = base + ((wave - base) % nWaves + nWaves) % nWaves; k < base + n; k += nWaves) {
+				const int2 e = g.redo[k - base];
+				const int sx = e.x % g.stripsX, sy = e.x / g.stripsX;
+				double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, ROE_INLINE>(a, g, sx, sy, dt, lane,
+					edges[waveInBlock], e.y);
+				stage_reduce(a, g, dtLocal, e.x, lane);
+			}
+			base += n;
 		}
-		base += n;
 	}
-	DevScalars *d = (batched ? a.grids[0] : a.g).dev;
-	unsigned last = 0u;
-	if (lane == 0) {
-		__threadfence();
-		last = atomicAdd(&d->redoDone, 1u) == (unsigned)(nWaves - 1) ? 1u : 0u;
-	}
-	if (__shfl(last, 0)) {
-		for (int gi = lane; gi < nGrids; gi += 64) (batched ? a.grids[gi] : a.g).dev->redoCount = 0u;
+	if (base == 0) return;     // the same for every wave of the launch
+	DevScalars *d = batched ? a.grids[0].dev : a.g.dev;
+	__syncthreads();
+	if (threadIdx.x == 0) lastBlock = atomicAdd(&d->redoDone, 1u) == gridDim.x - 1 ? 1u : 0u;
+	__syncthreads();
+	if (lastBlock && threadIdx.x < 64) {
+		for (int gi = lane; gi < nGrids; gi += 64) (batched ? a.grids[gi].dev : a.g.dev)->redoCount = 0u;
 		if (lane == 0) d->redoDone = 0u;
 	}
 }

@@ -328,14 +328,17 @@ int hydro_batch_autotune(hydro_batch *b, int trial_steps, int *rows_out, int *wa
 
 /* Continuous batching of solve requests (no counterpart in the reference; its
  * app loop steps a fixed set of solvers, hydro/app.lua:1573-1737). Requests
- * (configuration + initial state + step count) queue on submit; each tick
- * admits queued requests in arrival order while their device state fits the
- * HBM budget (ones that do not fit wait, smaller ones may pass them), advances
- * every resident request by up to steps_per_tick steps — requests of one kernel
- * configuration share persistent batched launches, configurations run on their
- * own streams and overlap — and retires finished ones: the final state is
- * copied to the host and the memory freed. A result equals the one of the same
- * request solved alone. The server runs the Runge-Kutta integrators. */
+ * (configuration + initial state + step count) queue on submit. Admission
+ * takes queued requests in arrival order while their device state fits the
+ * HBM budget and their packed state fits the pinned host staging (the smaller
+ * of the budget and 256 MiB); ones that do not fit wait, smaller ones may pass
+ * them. Requests of one kernel configuration form a pool, which advances on
+ * its own stream in chunks of up to steps_per_tick steps: the joining
+ * requests' uploads, persistent batched launches over every resident grid,
+ * and, for the requests the chunk finishes, a non-finite check on the device
+ * and the copy of the final state to the host. Pools run concurrently. A
+ * result equals the one of the same request solved alone. The server runs the
+ * Runge-Kutta integrators. */
 typedef struct hydro_server hydro_server;
 
 typedef struct hydro_server_info {
@@ -351,9 +354,14 @@ typedef struct hydro_server_info {
 int hydro_server_create(size_t hbm_budget_bytes, int steps_per_tick, hydro_server **out);
 int hydro_server_destroy(hydro_server *s);
 /* U: initial ghosted cons_t buffer as for hydro_upload_U. Fails when the
- * request alone needs more device memory than the budget. */
+ * request alone needs more device memory than the budget, or more host
+ * staging than the server has. */
 int hydro_server_submit(hydro_server *s, const hydro_cfg *cfg, const void *U, size_t bytes, int nsteps,
 	long long *id);
+/* Completes the chunks that have ended (results unpacked, memory released),
+ * admits, and starts the next chunk of every idle pool; it waits only when
+ * none of these can happen, for the chunk started first. n_completed: requests
+ * finished in this call. */
 int hydro_server_tick(hydro_server *s, int *n_completed);
 /* ids of finished requests whose results were not fetched yet (up to max) */
 int hydro_server_poll(hydro_server *s, long long *ids, int max, int *n);

@@ -3,8 +3,10 @@ expanding into vacuum (one- and two-sided vacuum Roe states, identity
 transforms below rhoMin, constrainU's density and pressure clamps), gas
 leaving a vacuum gap between two outgoing streams, and a cold gas (pressure
 below PMin, h < 1e-5 in the Roe average: Cs = 0), each in EXACT FP64, FAST
-FP64 and FP32, 1D, 2D along x and y, and 3D. The fast kernels handle these
-branches with wave-uniform tests (fv_euler.h), so these are their cases.
+FP64 and FP32, 1D, 2D along x and y, and 3D. The FAST Roe kernels without a
+flux limiter leave a strip, from its first row with an irregular interface
+on, to the redo kernel (fv_stage.h, ROE_FLAGGED); the other FAST kernels
+choose the Roe form per interface; so these are the cases of both paths.
 
 Each case checks that the oracle's run entered the branch it is named for
 (vacuum cells, clamped densities, clamped pressures), and the HIP result
@@ -152,3 +154,37 @@ def test_edge_branches(name, init, kw, steps, branch, mode):
           f'(oracle sensitivity {sens_t:.2e}), vacuum {seen_vac}, rho clamps {seen_rho_clamp}, P clamps {seen_p_clamp}')
     assert np.all(err <= bar), (err, bar)
     assert dt_rel <= max(tol, 10 * sens_t), (dt_rel, sens_t)
+
+
+@pytest.mark.parametrize('name,real', [('expansion-2d-x-roe', 'double'), ('streams-2d-y-roe', 'float')])
+def test_redo_same_bits_any_launch(name, real):
+    """The strips with vacuum interfaces go through the redo kernel, whose
+    rows start wherever the first irregular interface is. The Roe form is
+    chosen per interface and the arithmetic contracts only within a source
+    expression, so the result is the same to the bit at any rows per wave and
+    in the batched kernel (with its own redo launch) as in the plain one."""
+    from hydro.solver.batch import HipBatch
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as init_euler
+    _, init, kw, steps, _ = [c for c in CASES if c[0] == name][0]
+    kw = dict(kw, real=real, fastMath=True)
+    outs = []
+    for rows in (2, 5, 16):
+        s = HipFVSolver(**kw)
+        s.setLaunch(rows)
+        U0 = init_euler.to_ghosted(s.cfg, init(s.cfg)).astype(s.dtype)
+        s.resetState(U0)
+        s.update(steps)
+        outs.append((s.UBufObj.toCPU(), s.t))
+        del s
+    ckw = dict(oracle_kw(kw), precision=real, fast_math=True)
+    b = HipBatch([ckw, ckw])
+    b.upload(0, U0)
+    b.upload(1, U0[:, ::-1].copy())      # the same state mirrored in y: other rows irregular
+    b.constrainU()
+    b.setLaunch(3, 4)
+    b.update(steps)
+    outs.append((b.download(0), b.time(0)[0]))
+    for U, t in outs[1:]:
+        assert t == outs[0][1]
+        assert np.array_equal(U, outs[0][0])

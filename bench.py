@@ -12,7 +12,8 @@ Beside it on the same JSON line:
                 region, which holds only stage launches, over their count)
   hbm_regime    the same KH problem on 4096^2 (1.07 GB of FP64 state, beyond the
                 256 MB last-level cache), FP64 and FP32: the roofline fraction
-                with the stage kernel's traffic going to HBM
+                with the stage kernel's traffic going to HBM, beside what a
+                device-to-device copy of the same footprint reaches
   fp32          the same KH problem in the FP32 bulk form, and its relative L1
                 difference per conserved variable from the FP64 result
   fluxes        the same KH problem with the HLL, Rusanov, HLLC and Marquina
@@ -178,7 +179,47 @@ def hbm_leg(n, steps, warmup, fast, no_fp32):
             'roofline': {'bound': 'hbm', 'achieved': alg / (avg * 1e-3) / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                          'frac': alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 'algorithmic_bytes_per_launch': alg}}
         del s
+    res['device_copy_ceiling'] = copy_ceiling(n)
     return res
+
+
+def copy_ceiling(n, reps=10):
+    """What a plain device-to-device copy reaches on this device at the same
+    footprint (the 5-field FP64 state of one n x n grid): hipMemcpy D2D through
+    the HIP runtime the library is linked to (libamdhip64.so.7), timed with
+    events; GB/s of bytes read + written, and as a fraction of the 8 TB/s peak"""
+    import ctypes
+    hip = ctypes.CDLL('libamdhip64.so.7')
+    vp = ctypes.c_void_p
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    hip.hipFree.argtypes = [vp]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    nbytes = n * n * 5 * 8
+    src, dst, e0, e1 = vp(), vp(), vp(), vp()
+    if hip.hipMalloc(ctypes.byref(src), nbytes) or hip.hipMalloc(ctypes.byref(dst), nbytes):
+        return None
+    try:
+        assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+        assert hip.hipMemcpyAsync(dst, src, nbytes, 3, None) == 0          # hipMemcpyDeviceToDevice, warm
+        assert hip.hipEventRecord(e0, None) == 0
+        for _ in range(reps):
+            assert hip.hipMemcpyAsync(dst, src, nbytes, 3, None) == 0
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        gbs = 2 * nbytes * reps / (ms.value * 1e-3) / 1e9
+        return {'bytes_each_way': nbytes, 'GB/s': gbs, 'frac_of_peak': gbs / HBM_PEAK_GBS}
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        hip.hipFree(src)
+        hip.hipFree(dst)
 
 
 def batched_leg(count, steps, seed=0, accuracy_grids=8):
@@ -371,12 +412,22 @@ def main():
         sys.exit(2)
 
     if args.copy_calibration:
+        # kernels of known traffic for the FETCH_SIZE / WRITE_SIZE counters: a
+        # device copy of MIB MiB, and an elementwise negation of MIB MiB of
+        # FP64 and of FP32 whose operands start one element past a 16-byte
+        # boundary, so that each lane moves one 8- or 4-byte value per access,
+        # as the stage kernels do
         import torch
         a = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device='cuda')
         b = torch.empty_like(a)
         b.copy_(a)
+        for dt, w in ((torch.float64, 8), (torch.float32, 4)):
+            n = (args.copy_calibration << 20) // w
+            x = torch.ones(n + 1, dtype=dt, device='cuda')
+            y = torch.empty_like(x)
+            torch.neg(x[1:], out=y[1:])
         torch.cuda.synchronize()
-        del a, b
+        del a, b, x, y
 
     n = args.size
     fast = not args.exact_math

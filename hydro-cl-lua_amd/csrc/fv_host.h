@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hydro_hip.h"
@@ -532,9 +533,27 @@ static size_t ghosted_cells(const Grid &g) {
 	return c;
 }
 
+// Runs fn(r0, r1) over [0, n) split into contiguous ranges on up to 8 host
+// threads (one range, on the caller's thread, below `grain` items).
+template <typename F>
+static void parallel_ranges(long n, long grain, F fn) {
+	const long hw = std::max(1L, (long)std::thread::hardware_concurrency());
+	const long nt = std::max(1L, std::min({8L, hw, n / std::max(1L, grain)}));
+	if (nt <= 1) {
+		fn(0L, n);
+		return;
+	}
+	std::vector<std::thread> th;
+	th.reserve(nt - 1);
+	for (long t = 1; t < nt; ++t) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+	fn(0L, n / nt);
+	for (auto &x : th) x.join();
+}
+
 // AoS ghosted cons_t -> the device layout (q planes and nstat static planes of
 // ps values each, interior cells dense, the rest zero), into host memory the
-// caller provides; sets g.nvel and g.ePot from the state
+// caller provides; sets g.nvel and g.ePot from the state. Rows are packed on
+// several host threads.
 template <typename R>
 static void pack_interior(Grid &g, const R *aos, R *planes, R *stat) {
 	int gs[3], o[3];
@@ -542,34 +561,48 @@ static void pack_interior(Grid &g, const R *aos, R *planes, R *stat) {
 		gs[i] = i < g.dim ? g.n[i] + 2 * NG : 1;
 		o[i] = i < g.dim ? NG : 0;
 	}
-	std::fill(planes, planes + (size_t)g.Q * g.ps, R(0));
-	if (g.NSTAT) std::fill(stat, stat + (size_t)g.NSTAT * g.ps, R(0));
-	bool anyE = false;
-	if (g.cfg.eqn == HYDRO_EQN_EULER) g.ePot.assign(g.ncell, 0.);
-	for (int k = 0; k < g.n[2]; ++k)
-		for (int j = 0; j < g.n[1]; ++j)
-			for (int i = 0; i < g.n[0]; ++i) {
-				size_t gi = (size_t)(i + o[0]) + (size_t)gs[0] * ((j + o[1]) + (size_t)gs[1] * (k + o[2]));
-				size_t c = (size_t)i + (size_t)g.n[0] * (j + (size_t)g.n[1] * k);
-				const R *u = aos + gi * g.S;
-				for (int f = 0; f < g.Q; ++f) planes[(size_t)f * g.ps + c] = u[f];
-				for (int f = 0; f < g.NSTAT; ++f) stat[(size_t)f * g.ps + c] = u[g.Q + f];
-				if (g.cfg.eqn == HYDRO_EQN_EULER) {
-					g.ePot[c] = (double)u[5];
+	const bool euler = g.cfg.eqn == HYDRO_EQN_EULER;
+	if (euler) g.ePot.assign(g.ncell, 0.);
+	const long rows = (long)g.n[1] * g.n[2];
+	const int nx = g.n[0], Q = g.Q, NS = g.NSTAT, S = g.S, dim = g.dim;
+	const long ps = g.ps;
+	double *ePot = euler ? g.ePot.data() : nullptr;
+	// per-row flags (each row written by one thread), reduced afterwards
+	std::vector<char> rowE(euler ? rows : 0, 0), rowV(euler ? rows : 0, 0);
+	parallel_ranges(rows, 64, [&](long r0, long r1) {
+		for (long row = r0; row < r1; ++row) {
+			const int j = (int)(row % g.n[1]), k = (int)(row / g.n[1]);
+			const R *u = aos + ((size_t)o[0] + (size_t)gs[0] * ((j + o[1]) + (size_t)gs[1] * (k + o[2]))) * S;
+			const size_t c0 = (size_t)nx * (size_t)row;
+			bool anyE = false, nzv = false;
+			for (int i = 0; i < nx; ++i, u += S) {
+				const size_t c = c0 + i;
+				for (int f = 0; f < Q; ++f) planes[(size_t)f * ps + c] = u[f];
+				for (int f = 0; f < NS; ++f) stat[(size_t)f * ps + c] = u[Q + f];
+				if (euler) {
+					ePot[c] = (double)u[5];
 					anyE = anyE || u[5] != R(0);
+					for (int f = 1 + dim; f <= 3; ++f) nzv = nzv || u[f] != R(0);
 				}
 			}
+			if (euler) {
+				rowE[row] = anyE;
+				rowV[row] = nzv;
+			}
+		}
+	});
+	// each plane's cells past the interior (none while the planes are dense)
+	for (int f = 0; f < Q; ++f) std::fill(planes + (size_t)f * ps + g.ncell, planes + (size_t)(f + 1) * ps, R(0));
+	for (int f = 0; f < NS; ++f) std::fill(stat + (size_t)f * ps + g.ncell, stat + (size_t)(f + 1) * ps, R(0));
+	bool anyE = false, nzv = false;
+	for (long r = 0; r < (long)rowE.size(); ++r) {
+		anyE = anyE || rowE[r];
+		nzv = nzv || rowV[r];
+	}
 	if (!anyE) g.ePot.clear();
 	// Euler: momentum components beyond dim that are zero everywhere stay zero
 	// (their fluxes and wave amplitudes are exact zeros), so the kernels skip them
-	g.nvel = 3;
-	if (g.cfg.eqn == HYDRO_EQN_EULER && g.dim < 3) {
-		bool zero = true;
-		for (int f = 1 + g.dim; f <= 3 && zero; ++f)
-			for (long c = 0; c < g.ps; ++c)
-				if (planes[(size_t)f * g.ps + c] != R(0)) { zero = false; break; }
-		if (zero) g.nvel = g.dim;
-	}
+	g.nvel = euler && dim < 3 && !nzv ? dim : 3;
 }
 
 template <typename R>
@@ -577,8 +610,9 @@ static void unpack_with_ghosts(const Grid &g, const R *planes, const R *stat, R 
 	int gs[3];
 	for (int i = 0; i < 3; ++i) gs[i] = i < g.dim ? g.n[i] + 2 * NG : 1;
 	const hydro_cfg &c = g.cfg;
-	for (int k = 0; k < gs[2]; ++k)
-		for (int j = 0; j < gs[1]; ++j)
+	parallel_ranges((long)gs[1] * gs[2], 64, [&](long r0, long r1) {
+		for (long row = r0; row < r1; ++row) {
+			const int j = (int)(row % gs[1]), k = (int)(row / gs[1]);
 			for (int i = 0; i < gs[0]; ++i) {
 				int cc[3] = {i - (g.dim > 0 ? NG : 0), j - (g.dim > 1 ? NG : 0), k - (g.dim > 2 ? NG : 0)};
 				HostMap m[3];
@@ -600,6 +634,8 @@ static void unpack_with_ghosts(const Grid &g, const R *planes, const R *stat, R 
 					u[f] = v;
 				}
 			}
+		}
+	});
 }
 
 template <typename R>

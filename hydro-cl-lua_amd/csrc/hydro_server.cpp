@@ -1,11 +1,11 @@
 // hydro_server.cpp — continuous-batching solve server.
 //
 // Requests (a solver configuration, an initial state and a step count) queue on
-// submit. Admission takes queued requests in arrival order while their device
-// state fits the HBM budget and their host staging fits the pinned staging
-// area (requests that do not fit wait; smaller ones behind them may pass):
-// the state is packed into the device layout in pinned memory and the request
-// joins the pool of its kernel configuration. Each pool advances on its own
+// submit, their state packed into the device layout. Admission takes queued
+// requests in arrival order while their device state fits the HBM budget and
+// their host staging fits the pinned staging area (requests that do not fit
+// wait; smaller ones behind them may pass): the packed state is copied into
+// pinned memory and the request joins the pool of its kernel configuration. Each pool advances on its own
 // stream in chunks of up to steps_per_tick steps: a chunk uploads the pool's
 // joining requests, runs the persistent batched stage launches over every
 // resident grid, and, for each request the chunk finishes, a non-finite check
@@ -74,7 +74,7 @@ enum { QUEUED = 0, JOINING = 1, RESIDENT = 2, RETIRING = 3, FINISHED = 4 };
 struct Request {
 	long long id = 0;
 	hydro_cfg cfg;
-	std::vector<char> U0;      // the caller's state until it is packed
+	std::vector<char> packed;  // the state in the device layout, from submit to admission
 	int nsteps = 0;
 	int remaining = 0;         // steps not yet launched
 	int state = QUEUED;
@@ -213,7 +213,13 @@ extern "C" int hydro_server_submit(hydro_server *s, const hydro_cfg *cfg, const 
 	r.stageBytes = staging_bytes(r.g);
 	if (r.stageBytes > s->pinned.size)
 		return fail("request needs %zu B of host staging, more than the server's %zu B", r.stageBytes, s->pinned.size);
-	r.U0.assign((const char *)U, (const char *)U + bytes);
+	// packed once, here, into the device layout (this also sets nvel, which
+	// selects the pool)
+	r.packed.resize(plane_bytes(r.g));
+	char *planes = r.packed.data();
+	char *stat = planes + (size_t)r.g.Q * r.g.ps * r.g.elem;
+	if (r.g.elem == 8) pack_interior<double>(r.g, (const double *)U, (double *)planes, (double *)stat);
+	else pack_interior<float>(r.g, (const float *)U, (float *)planes, (float *)stat);
 	r.nsteps = r.remaining = nsteps;
 	s->queued.push_back(r.id);
 	*id = r.id;
@@ -239,13 +245,18 @@ static int pool_for(hydro_server *s, const Request &r) {
 }
 
 // Queued requests that fit the device arena and the pinned staging, in arrival
-// order (ones that do not fit wait, smaller ones may pass): the state is packed
-// into the device layout in pinned memory (host work only) and the request
-// joins its pool. A request that fails a step gets its blocks back and stays
-// queued, ahead of those not yet looked at (the queue order is kept).
+// order (ones that do not fit wait, smaller ones may pass): the packed state
+// is copied into pinned memory (host work only) and the request joins its
+// pool. One call copies at most about ADMIT_BYTES (at least one
+// request), so that pools start while later ticks copy the rest. A
+// request that fails a step gets its blocks back and stays queued, ahead of
+// those not yet looked at (the queue order is kept).
+constexpr size_t ADMIT_BYTES = (size_t)64 << 20;
+
 static int admit(hydro_server *s, bool *moved) {
 	std::deque<long long> still;
-	while (!s->queued.empty()) {
+	size_t copied = 0;
+	while (!s->queued.empty() && copied < ADMIT_BYTES) {
 		long long id = s->queued.front();
 		Request &r = s->reqs.at(id);
 		long off = s->arena.alloc(r.bytes);
@@ -260,9 +271,10 @@ static int admit(hydro_server *s, bool *moved) {
 		r.upOff = st;
 		grid_bind(r.g, s->arena.base + r.off);
 		char *stage = s->pinned.base + st;
-		char *stat = stage + (size_t)r.g.Q * r.g.ps * r.g.elem;
-		if (r.g.elem == 8) pack_interior<double>(r.g, (const double *)r.U0.data(), (double *)stage, (double *)stat);
-		else pack_interior<float>(r.g, (const float *)r.U0.data(), (float *)stage, (float *)stat);
+		const char *src = r.packed.data();
+		parallel_ranges((long)r.packed.size(), (long)1 << 20, [&](long b0, long b1) {
+			std::memcpy(stage + b0, src + b0, (size_t)(b1 - b0));
+		});
 		int p = pool_for(s, r);
 		if (p < 0) {
 			s->arena.release(r.off, r.bytes);
@@ -272,14 +284,16 @@ static int admit(hydro_server *s, bool *moved) {
 			return -1;
 		}
 		s->queued.pop_front();
-		r.U0.clear();
-		r.U0.shrink_to_fit();
+		copied += r.stageBytes;
+		r.packed.clear();
+		r.packed.shrink_to_fit();
 		r.pool = p;
 		r.state = JOINING;
 		s->pools[p].joining.push_back(id);
 		*moved = true;
 	}
-	s->queued.swap(still);
+	// those that did not fit go back ahead of the ones not looked at
+	for (auto it = still.rbegin(); it != still.rend(); ++it) s->queued.push_front(*it);
 	return 0;
 }
 

@@ -160,6 +160,21 @@ def timed_steps(s, warmup, steps):
     return time.perf_counter() - t0, kt.total_ms
 
 
+def committed_traffic(key):
+    """HBM bytes per stage launch of a workload and where they come from: the
+    rocprofv3 FETCH_SIZE and WRITE_SIZE passes committed under profiles/
+    (counters cannot be read inside the timed run)"""
+    tpath = os.path.join(REPO, 'profiles', 'stage_kernel_traffic.json')
+    try:
+        tj = json.load(open(tpath))
+    except (OSError, ValueError):
+        return None, None
+    entry = tj.get(key)
+    if isinstance(entry, dict):
+        return entry.get('bytes_per_launch'), entry.get('source')
+    return entry, tj.get('source')
+
+
 def hbm_leg(n, steps, warmup, fast, no_fp32):
     """The headline workload on a grid whose state does not fit the 256 MB
     last-level cache (4096^2: U^n and U^1 are 1.07 GB in FP64), so that the
@@ -178,6 +193,8 @@ def hbm_leg(n, steps, warmup, fast, no_fp32):
             'rows_per_wave': s.getLaunch()[0],
             'roofline': {'bound': 'hbm', 'achieved': alg / (avg * 1e-3) / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                          'frac': alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 'algorithmic_bytes_per_launch': alg}}
+        tr, src = committed_traffic(f'kh{n}_{real}')
+        res['f64' if real == 'double' else 'f32']['roofline'].update({'traffic': tr, 'traffic_source': src})
         del s
     res['device_copy_ceiling'] = copy_ceiling(n)
     return res
@@ -449,15 +466,7 @@ def main():
 
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     # passes of this same command (counters cannot be read inside the run)
-    traffic, traffic_src = None, None
-    tpath = os.path.join(REPO, 'profiles', 'stage_kernel_traffic.json')
-    if os.path.exists(tpath):
-        try:
-            tj = json.load(open(tpath))
-            traffic = tj.get(f'kh{n}_{args.real}')
-            traffic_src = tj.get('source')
-        except (OSError, ValueError):
-            traffic = None
+    traffic, traffic_src = committed_traffic(f'kh{n}_{args.real}')
 
     out = {
         'metric': 'Gcell-updates/s (2D Euler Roe+PLM) + batched solves/s & p99 latency vs plain',

@@ -360,13 +360,17 @@ def blast3d_leg(n, steps, seed=0):
     }
 
 
-def serving_leg(count, seed=0, steps_per_tick=20, budget=32 << 30):
+def serving_leg(count, seed=0, steps_per_tick=20, budget=32 << 30, closed_only=False):
     from hydro import workloads as wl
     from hydro.serve import serve, serve_sequential
     reqs = wl.serving_mix(count, seed=seed)
     cu = sum(r.cell_updates() for r in reqs)
     lat_c, mk_c, _ = serve(reqs, budget, steps_per_tick, closed_loop=True)
     capacity = count / mk_c
+    if closed_only:
+        return {'closed_loop_solves_per_s': capacity, 'closed_loop_gcell_updates_per_s': cu / mk_c / 1e9,
+                'closed_loop_makespan_s': mk_c, 'cell_updates': cu,
+                'note': 'closed-loop served run only (--serve-closed-only)'}
     rate = .8 * capacity
     reqs_p = wl.serving_mix(count, seed=seed, rate=rate)
     lat, mk, _ = serve(reqs_p, budget, steps_per_tick)
@@ -420,6 +424,8 @@ def main():
     ap.add_argument('--batch-grids', type=int, default=512)
     ap.add_argument('--batch-steps', type=int, default=300)
     ap.add_argument('--serve-requests', type=int, default=120)
+    ap.add_argument('--serve-closed-only', action='store_true',
+                    help='serving leg: the closed-loop served run only (for profiling)')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='first run one device copy of this many MiB (counter calibration for profiling)')
     args = ap.parse_args()
@@ -534,7 +540,7 @@ def main():
     if not args.no_3d:
         out['blast3d'] = blast3d_leg(args.size3d, args.steps3d)
     if not args.no_serve:
-        out['serving'] = serving_leg(args.serve_requests)
+        out['serving'] = serving_leg(args.serve_requests, closed_only=args.serve_closed_only)
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(args.cpu_seconds)
     else:

@@ -177,12 +177,12 @@ struct Grid {
 };
 
 // Stage kernels (plain and batched) that leave strips with an irregular Roe
-// interface to a redo kernel (fv_stage.h, ROE_FLAGGED): FAST, Euler, Roe
-// without the entropy fix and without a flux limiter, 1D and 2D
+// interface to a redo kernel (fv_stage.h, ROE_FLAGGED, redo_form): FAST,
+// Euler, Roe without the entropy fix and without a flux limiter, 2D
 static inline bool redo_kernels(const hydro_cfg &c) {
 	const bool lim = c.plm == HYDRO_PLM_NONE && c.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
 	return c.fast_math && c.eqn == HYDRO_EQN_EULER && c.flux == HYDRO_FLUX_ROE && !c.roe_entropy_fix && !lim
-		&& c.dim <= 2;
+		&& c.dim == 2;
 }
 
 // Blocks of the redo kernel, whose waves stride over the redo lists.
@@ -286,7 +286,8 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 	const long cus = device_cus();
 	int best = 4;
 	double bestT = -1;
-	for (int r = 4; r <= EDGE_ROWS; ++r) {
+	// at most 8 rows: longer strips measured no faster at 2048^2 and 4096^2
+	for (int r = 4; r <= 8; ++r) {
 		long blocks = (stripsX * ((ny + r - 1) / r) + 3) / 4;
 		long L = (blocks + cus - 1) / cus;
 		double t = (double)L * (r + warm + 1.5) / eff[std::min(L, 3L)];

@@ -448,16 +448,25 @@ __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface
 // the collected form only and reports per lane whether the interface is
 // regular, so that the stage kernel leaves a strip from its first row with an
 // irregular interface on to the redo kernel (stage_kernel_redo), which runs
-// ROE_INLINE. Either way each interface's form depends on its own states only,
-// so a grid advances to the same bits whatever its strips' heights and
-// whether it runs alone or batched. Keeping the general form out of the stage
-// kernel keeps its registers at the collected form's (about 125 VGPRs FP64: 4
-// waves per SIMD, against 186 with both forms).
+// ROE_INLINE (1D and 3D kernels take ROE_INLINE throughout). Either way each
+// interface's form depends on its own states only, so a grid advances to the
+// same bits whatever its strips' heights and whether it runs alone or
+// batched. Keeping the general form out of the 2D stage kernel keeps its
+// registers at the collected form's (127 VGPRs FP64: 4 waves per SIMD,
+// against 186 with both forms).
 enum { ROE_GENERAL = 0, ROE_INLINE = 1, ROE_FLAGGED = 2 };
 
 template <class E>
 __host__ __device__ constexpr bool roe_collected(bool lim) {
 	return E::FASTM && E::FLUXID == HYDRO_FLUX_ROE && E::Q == 5 && E::NW == 5 && !lim;
+}
+
+// The stage kernels that take ROE_FLAGGED and a redo launch: 2D. A 1D strip
+// is one row, so the general form costs it few registers (94 VGPRs FP64
+// against 74) and saves a launch per stage; 1D kernels take ROE_INLINE.
+template <class E, int DIM>
+__host__ __device__ constexpr bool redo_form(bool lim) {
+	return roe_collected<E>(lim) && DIM == 2;
 }
 
 // ------------------------------------------------- one interface, no limiter
@@ -1047,14 +1056,14 @@ __device__ __forceinline__ void stage_reduce(const StageArgs<R> &a, const GridDe
 }
 
 // ---------------------------------------------------------- the kernels
-// Plain form: one grid, one wave per strip. The FAST kernels of the Euler
+// Plain form: one grid, one wave per strip. The 2D FAST kernels of the Euler
 // equations with the Roe flux and no flux limiter take the collected Roe form
 // only (ROE_FLAGGED) and leave any strip with an irregular interface, from
 // that row on, to stage_kernel_redo, launched after them on the same stream.
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
 __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 {
-	constexpr int RM = roe_collected<EQ<R>>(LIM) ? ROE_FLAGGED : ROE_INLINE;
+	constexpr int RM = redo_form<EQ<R>, DIM>(LIM) ? ROE_FLAGGED : ROE_INLINE;
 	const int lane = threadIdx.x & 63;
 	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
 	const int waveInBlock = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1138,7 +1147,7 @@ extern __shared__ __align__(16) unsigned char batch_lds[];
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM>
 __global__ void __launch_bounds__(256) stage_kernel_batch(const StageArgs<R> a)
 {
-	constexpr int RM = roe_collected<EQ<R>>(LIM) ? ROE_FLAGGED : ROE_INLINE;
+	constexpr int RM = redo_form<EQ<R>, DIM>(LIM) ? ROE_FLAGGED : ROE_INLINE;
 	const int lane = threadIdx.x & 63;
 	int blk = a.xcdRemap ? xcd_block(blockIdx.x, a.nBlocks) : (int)blockIdx.x;
 	const int waveInBlock = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

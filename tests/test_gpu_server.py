@@ -147,3 +147,45 @@ def test_served_fp32_results_against_fp64_oracle():
         print(r.kind, r.kw['size'], r.steps, err.max())
         assert np.all(err <= 1e-5), (i, r.kind, err)
         assert abs(t - o.t) <= 1e-5 * o.t
+
+
+def test_pinned_staging_limits_admission_and_copy_out():
+    """requests whose packed states fill the server's pinned staging (256 MiB
+    under a larger budget): admission waits for staging as it waits for
+    memory, finished requests wait for staging to copy out, and results do
+    not change; a request whose packed state alone exceeds the staging is
+    refused at submit"""
+    from hydro.serve import Server, serve_sequential
+    from hydro import workloads as wl
+    from hydro._abi import HydroError
+    rng = np.random.default_rng(5)
+    reqs = [wl.Request('euler2d', wl.quadrant_2d_kw(1300, 1280 + 10 * i, real='double', fast=True),
+                       int(rng.integers(2, 6)), int(rng.integers(0, 2 ** 31))) for i in range(6)]
+    srv = Server(budget_bytes=1 << 30, steps_per_tick=2)
+    big = wl.Request('euler2d', wl.quadrant_2d_kw(2900, 2900, real='double', fast=True), 1, 0)
+    with pytest.raises(HydroError, match='staging'):
+        srv.submit(big.kw, np.zeros(big_shape(big.kw)), 1)
+    ids = {srv.submit(r.kw, r.initial_state(), r.steps): i for i, r in enumerate(reqs)}
+    results = {}
+    queued_seen = False
+    for _ in range(10000):
+        info = srv.info()
+        if info['queued'] + info['resident'] == 0:
+            break
+        srv.tick()
+        queued_seen = queued_seen or srv.info()['queued'] > 0
+        for rid in srv.poll():
+            results[ids[rid]] = srv.result(rid)
+    assert len(results) == len(reqs)
+    assert queued_seen, 'the staging was expected to hold back some requests'
+    _, _, alone = serve_sequential(reqs, keep_results=True, closed_loop=True)
+    for i in range(len(reqs)):
+        assert results[i][2] == 0
+        assert results[i][1] == alone[i][1]
+        assert np.array_equal(results[i][0], alone[i][0]), i
+
+
+def big_shape(kw):
+    """the ghosted cons_t shape of a request's state"""
+    from hydro._cfg import make_cfg, cfg_ghost_shape
+    return cfg_ghost_shape(make_cfg(**kw))

@@ -312,99 +312,152 @@ struct EulerT {
 			F[4] = (U[4] + W.P) * vn;
 		}
 
-		// The Roe flux F = (F(UL) + F(UR)) / 2 - R |Lambda| L (UR - UL) / 2
-		// (roe.cl:17-163 without a flux limiter) of an interface whose states
-		// are regular: both densities at least max(1e-5, rhoMin) and the Roe
-		// average's h = hTotal - v^2/2 at least 1e-5, so that none of
+		// The Roe quantities of an interface whose states are regular: both
+		// densities at least max(1e-5, rhoMin) and the Roe average's
+		// h = hTotal - v^2/2 at least 1e-5, so that none of
 		// eigen_forInterface's vacuum and cold branches (euler.cl:357-430) and
 		// no identity transform (euler.cl:432-542) applies. FAST policy only:
-		// the same function of the two states as the reference's, with its
+		// the same functions of the two states as the reference's, with the
 		// algebra collected:
 		//   Roe weight of the left state  wl = 1 / (1 + sqrt(rhoR / rhoL)),
 		//   v~ = vR + wl (vL - vR), hTotal~ likewise,
 		//   L dU through dp = g1 (dE - v~.dm + v~^2/2 drho) and
 		//   c (vn drho - dm_n): y0,4 = (dp +- c (vn drho - dm_n)) / (2 c^2),
 		//   y1 = drho - dp / c^2, tangential y = dm_t - v_t drho,
-		//   R in the matching factored form,
 		// c and 1 / c from one reciprocal square root. Computed whatever the
-		// states; returns whether this lane's interface is regular (F holds
-		// the result only then: otherwise the caller takes the general form,
-		// which has every branch).
+		// states; ok tells whether this lane's interface is regular (the
+		// values are used only then: otherwise the caller takes the general
+		// form, which has every branch).
+		struct Collected {
+			R vL[3], vR[3], PL, PR, EPL, EPR;   // the states' velocities, pressures, E + P
+			R v[3], H, hq, c;                   // Roe velocity, hTotal, v^2 / 2, sound speed
+			R y[5];                             // L (UR - UL)
+			bool ok;
+		};
 		template <int S>
-		__device__ static __forceinline__ bool roe_regular_lane(const Par &p, const R *UL, const R *UR, R *F) {
-			static_assert(FAST, "the collected Roe flux belongs to the FAST policy");
+		__device__ static __forceinline__ Collected roe_collect(const Par &p, const R *UL, const R *UR) {
+			static_assert(FAST, "the collected Roe arithmetic belongs to the FAST policy");
 			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
+			Collected q;
 			const R rhoFloor = fmax(R(1e-5), p.rhoMin);
 			const bool okRho = UL[0] >= rhoFloor && UR[0] >= rhoFloor;   // NaN: not regular
 			const R iL = M::rcp(UL[0]), iR = M::rcp(UR[0]);
-			R vL[3], vR[3];
 #pragma unroll
 			for (int i = 0; i < 3; ++i) {
-				vL[i] = i < NVEL ? UL[1 + i] * iL : R(0);
-				vR[i] = i < NVEL ? UR[1 + i] * iR : R(0);
+				q.vL[i] = i < NVEL ? UL[1 + i] * iL : R(0);
+				q.vR[i] = i < NVEL ? UR[1 + i] * iR : R(0);
 			}
-			const R PL = p.g1 * (UL[4] - R(.5) * dot3(UL + 1, vL));
-			const R PR = p.g1 * (UR[4] - R(.5) * dot3(UR + 1, vR));
-			const R EPL = UL[4] + PL, EPR = UR[4] + PR;
-			const R HL = EPL * iL, HR = EPR * iR;
+			q.PL = p.g1 * (UL[4] - R(.5) * dot3(UL + 1, q.vL));
+			q.PR = p.g1 * (UR[4] - R(.5) * dot3(UR + 1, q.vR));
+			q.EPL = UL[4] + q.PL;
+			q.EPR = UR[4] + q.PR;
+			const R HL = q.EPL * iL, HR = q.EPR * iR;
 			const R wl = M::rcp(R(1) + M::sqrt_pos(UR[0] * iL));
-			R v[3];
 #pragma unroll
-			for (int i = 0; i < 3; ++i) v[i] = i < NVEL ? vR[i] + wl * (vL[i] - vR[i]) : R(0);
-			const R H = HR + wl * (HL - HR);
-			const R hq = R(.5) * lenSq3(v);
-			const R h = H - hq;
-			const bool ok = okRho && h >= R(1e-5);
+			for (int i = 0; i < 3; ++i) q.v[i] = i < NVEL ? q.vR[i] + wl * (q.vL[i] - q.vR[i]) : R(0);
+			q.H = HR + wl * (HL - HR);
+			q.hq = R(.5) * lenSq3(q.v);
+			const R h = q.H - q.hq;
+			q.ok = okRho && h >= R(1e-5);
 			// c = sqrt(g1 h) and 1 / c: two Goldschmidt steps from v_rsq
 			const R c2 = p.g1 * h;
 			R s, hh;
 			M::sqrt_and_half_rsqrt(c2, s, hh);
 			const R c = s, ic2 = R(4) * hh * hh;   // 1 / c^2
+			q.c = c;
 			// L (UR - UL)
 			R X[5];
 #pragma unroll
 			for (int j = 0; j < 5; ++j) X[j] = UR[j] - UL[j];
-			R t = X[4] + hq * X[0];
+			R t = X[4] + q.hq * X[0];
 #pragma unroll
 			for (int i = 0; i < 3; ++i)
-				if (i < NVEL) t = t - v[i] * X[1 + i];
+				if (i < NVEL) t = t - q.v[i] * X[1 + i];
 			const R dp = p.g1 * t;
-			const R vn = v[A];
+			const R vn = q.v[A];
 			const R cB = c * (vn * X[0] - X[1 + A]);
 			const R hic2 = R(.5) * ic2;
-			const R y0 = (dp + cB) * hic2, y4 = (dp - cB) * hic2;
-			const R y1 = X[0] - dp * ic2;
-			const R yB = B < NVEL ? X[1 + B] - v[B] * X[0] : R(0);
-			const R yC = C < NVEL ? X[1 + C] - v[C] * X[0] : R(0);
+			q.y[0] = (dp + cB) * hic2;
+			q.y[4] = (dp - cB) * hic2;
+			q.y[1] = X[0] - dp * ic2;
+			q.y[2] = B < NVEL ? X[1 + B] - q.v[B] * X[0] : R(0);
+			q.y[3] = C < NVEL ? X[1 + C] - q.v[C] * X[0] : R(0);
+			return q;
+		}
+
+		// The Roe flux F = (F(UL) + F(UR)) / 2 - R |Lambda| L (UR - UL) / 2
+		// (roe.cl:17-163 without a flux limiter) of a regular interface, from
+		// roe_collect, with R in the matching factored form. Returns whether
+		// this lane's interface is regular (F holds the result only then).
+		template <int S>
+		__device__ static __forceinline__ bool roe_regular_lane(const Par &p, const R *UL, const R *UR, R *F) {
+			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
+			const Collected q = roe_collect<S>(p, UL, UR);
+			const R c = q.c, vn = q.v[A];
 			// |Lambda| L dU
 			const R an = fabs(vn);
-			const R f0 = fabs(vn - c) * y0, f4 = fabs(vn + c) * y4;
-			const R f1 = an * y1, fB = an * yB, fC = an * yC;
+			const R f0 = fabs(vn - c) * q.y[0], f4 = fabs(vn + c) * q.y[4];
+			const R f1 = an * q.y[1], fB = an * q.y[2], fC = an * q.y[3];
 			// R (|Lambda| L dU)
 			const R z0 = f0 + f1 + f4;
 			const R d40 = f4 - f0;
 			R zm[3];
 			zm[A] = vn * z0 + c * d40;
-			zm[B] = B < NVEL ? v[B] * z0 + fB : R(0);
-			zm[C] = C < NVEL ? v[C] * z0 + fC : R(0);
-			R z4 = H * (f0 + f4) + (c * vn) * d40 + hq * f1;
-			if constexpr (B < NVEL) z4 = z4 + v[B] * fB;
-			if constexpr (C < NVEL) z4 = z4 + v[C] * fC;
+			zm[B] = B < NVEL ? q.v[B] * z0 + fB : R(0);
+			zm[C] = C < NVEL ? q.v[C] * z0 + fC : R(0);
+			R z4 = q.H * (f0 + f4) + (c * vn) * d40 + q.hq * f1;
+			if constexpr (B < NVEL) z4 = z4 + q.v[B] * fB;
+			if constexpr (C < NVEL) z4 = z4 + q.v[C] * fC;
 			// (F(UL) + F(UR) - z) / 2
-			const R vnL = vL[A], vnR = vR[A];
+			const R vnL = q.vL[A], vnR = q.vR[A];
 			F[0] = R(.5) * ((UL[1 + A] + UR[1 + A]) - z0);
 #pragma unroll
 			for (int i = 0; i < 3; ++i) {
 				if (i < NVEL || i == A) {
 					R fs = UL[1 + i] * vnL + UR[1 + i] * vnR;
-					if (i == A) fs = fs + (PL + PR);
+					if (i == A) fs = fs + (q.PL + q.PR);
 					F[1 + i] = R(.5) * (fs - zm[i]);
 				} else {
 					F[1 + i] = R(0);
 				}
 			}
-			F[4] = R(.5) * ((EPL * vnL + EPR * vnR) - z4);
-			return ok;
+			F[4] = R(.5) * ((q.EPL * vnL + q.EPR * vnR) - z4);
+			return q.ok;
+		}
+
+		// The interface quantities the flux-limited Roe flux (roe.cl:17-163)
+		// keeps of a regular interface, from roe_collect: the Roe state as
+		// eigen_forInterface gives it (rho only tells the transforms that the
+		// state is above rhoMin), L (UR - UL), and the mean of the two
+		// physical fluxes (euler.cl:273-291). Returns whether this lane's
+		// interface is regular (the outputs hold the results only then).
+		template <int S>
+		__device__ static __forceinline__ bool roe_prep_lane(const Par &p, const R *UL, const R *UR, Eig &e, R *dW,
+			R *Favg) {
+			constexpr int A = S;
+			const Collected q = roe_collect<S>(p, UL, UR);
+			e.rho = UL[0];
+#pragma unroll
+			for (int i = 0; i < 3; ++i) e.v[i] = q.v[i];
+			e.hTotal = q.H;
+			e.Cs = q.c;
+			e.vSq = R(2) * q.hq;
+#pragma unroll
+			for (int j = 0; j < 5; ++j) dW[j] = q.y[j];
+			const R vnL = q.vL[A], vnR = q.vR[A];
+			Favg[0] = R(.5) * (UL[1 + A] + UR[1 + A]);
+#pragma unroll
+			for (int i = 0; i < 3; ++i) {
+				if (i < NVEL || i == A) {
+					R fs = UL[1 + i] * vnL + UR[1 + i] * vnR;
+					if (i == A) fs = fs + (q.PL + q.PR);
+					Favg[1 + i] = R(.5) * fs;
+				} else {
+					Favg[1 + i] = R(0);
+				}
+			}
+			Favg[4] = R(.5) * (q.EPL * vnL + q.EPR * vnR);
+			return q.ok;
 		}
 
 		// eigenWaveCode (euler.lua:309-325)

@@ -375,18 +375,26 @@ template <typename R, class E>
 struct Iface {
 	typename E::Eig e;
 	R dW[E::NW];
-	R lambda[E::NW];
 	R Favg[E::Q];
 };
 
+// The FAST kernels of the Euler equations with the Roe flux prepare a regular
+// interface in the collected form (EulerT::roe_prep_lane) and the others in
+// the general one, lane by lane.
+template <class E>
+__host__ __device__ constexpr bool roe_collected_prep() {
+	return E::FASTM && E::FLUXID == HYDRO_FLUX_ROE && E::Q == 5 && E::NW == 5;
+}
+
+// eigen_forInterface, eigen_leftTransform of UR - UL and the mean of the
+// physical fluxes, with every branch of the reference (euler.cl:347-542)
 template <typename R, class E, int S>
-__device__ __forceinline__ void iface_prep(const typename E::Par &p, const Cell<R, E> &UL,
+__device__ __forceinline__ void iface_prep_general(const typename E::Par &p, const Cell<R, E> &UL,
 	const Cell<R, E> &UR, Iface<R, E> &I)
 {
 	auto WL = E::prim(p, UL.v);
 	auto WR = E::prim(p, UR.v);
 	I.e = E::eigen(p, UL.v, UR.v, WL, WR);
-	E::template waves<S>(p, I.e, I.lambda);
 	R dU[E::Q];
 #pragma unroll
 	for (int j = 0; j < E::Q; ++j) dU[j] = UR.v[j] - UL.v[j];
@@ -398,15 +406,30 @@ __device__ __forceinline__ void iface_prep(const typename E::Par &p, const Cell<
 	for (int j = 0; j < E::Q; ++j) I.Favg[j] = R(.5) * (FL[j] + FR[j]);
 }
 
+template <typename R, class E, int S>
+__device__ __forceinline__ void iface_prep(const typename E::Par &p, const Cell<R, E> &UL,
+	const Cell<R, E> &UR, Iface<R, E> &I)
+{
+	if constexpr (roe_collected_prep<E>()) {
+		const bool ok = E::template roe_prep_lane<S>(p, UL.v, UR.v, I.e, I.dW, I.Favg);
+		// the irregular lanes overwrite their own results (the wave-wide
+		// tests inside the general form only skip work none of their lanes needs)
+		if (!ok) iface_prep_general<R, E, S>(p, UL, UR, I);
+	} else {
+		iface_prep_general<R, E, S>(p, UL, UR, I);
+	}
+}
+
 // roe.cl:91-160
 template <typename R, class E, int S, bool LIM, int FLIM>
 __device__ __forceinline__ void iface_flux(const typename E::Par &p, const Iface<R, E> &I,
 	const R *dWL, const R *dWR, R dt_dx, int lim, R *F)
 {
-	R fluxEig[E::NW];
+	R fluxEig[E::NW], lam[E::NW];
+	E::template waves<S>(p, I.e, lam);
 #pragma unroll
 	for (int j = 0; j < E::NW; ++j) {
-		R lambda = I.lambda[j];
+		R lambda = lam[j];
 		if constexpr (E::FLUXID == FLUX_ROE_ENTROPY_FIX) {
 			// Harten's entropy fix (roe.cl:83-106): |lambda| below dws is
 			// replaced by (lambda^2/dws + dws)/2 with OpenCL's sign (0 at 0)
@@ -491,7 +514,7 @@ __device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell
 			if (E::template roe_regular_lane<S>(p, UL.v, UR.v, F)) return;
 		}
 		Iface<R, E> I;
-		iface_prep<R, E, S>(p, UL, UR, I);
+		iface_prep_general<R, E, S>(p, UL, UR, I);
 		iface_flux<R, E, S, false, 0>(p, I, nullptr, nullptr, dt_dx, lim, F);
 	} else {
 		using M = typename E::M;

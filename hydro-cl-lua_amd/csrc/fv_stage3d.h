@@ -407,7 +407,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 
 // One block of W waves per (strip, row block, plane chunk); grid = items.
 template <typename R, template <typename> class EQ, bool PLM, bool LIM, int FLIM, int SLIM>
-__global__ void __launch_bounds__(WAVE * Block3<R>::W) stage_kernel3d(const StageArgs<R> a)
+__global__ void __launch_bounds__(WAVE * Block3<R>::W, 3) stage_kernel3d(const StageArgs<R> a)
 {
 	constexpr int W = Block3<R>::W;
 	__shared__ Lds3<R, EQ<R>, W> sh;

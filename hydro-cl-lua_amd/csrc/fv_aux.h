@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(256) nonfinite_kernel(const R *U, const R *sta
 		for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x)
 			bad = bad || !__builtin_isfinite(p[c]);
 	}
-	if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+	if (wave_any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
 }
 
 // calcDT (eqn.lua:1187-1224 + calcDT.cl:38-73) and reduceMin: per-cell dt in

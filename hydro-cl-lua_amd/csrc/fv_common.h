@@ -29,13 +29,16 @@ constexpr int EDGE_ROWS = WAVE / 2;// rows per wave item at most (2D): the strip
                                    // interfaces of each row take a lane per side
 constexpr int EDGE_PLANES = 16;    // planes per block chunk at most (3D)
 
-// 3D stage kernel: waves per block (rows of a strip that share one plane
-// through LDS). Four, so that two blocks fit per CU at the kernel's register
-// use and one block's barrier waits are covered by the other's work (8 waves
-// per block ran 30% slower at 256^3 FP32).
+// 3D stage kernel: a block is W row waves (rows of a strip that share one
+// plane through LDS) and one halo wave that loads the rows outside the block
+// and computes the y interfaces no row owns, so that the row waves do equal
+// work between barriers. Four waves, so that three blocks fit per CU at the
+// kernel's register use and one block's barrier waits are covered by the
+// others' work.
 template <typename R>
 struct Block3 {
-	static constexpr int W = 4;
+	static constexpr int W = 3;
+	static constexpr int WAVES = W + 1;
 };
 
 // Device-resident per-solver scalars: dt minima rotate through three slots so

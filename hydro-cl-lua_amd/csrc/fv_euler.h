@@ -97,7 +97,7 @@ struct EulerT {
 			W.P = p.g1 * (U[4] - EKin);
 			// below rhoMin the reference's P is 0 (its EKin 0 only feeds P);
 			// wave-uniform test, so states above rhoMin pay no select
-			if (__any(U[0] < p.rhoMin)) W.P = U[0] < p.rhoMin ? R(0) : W.P;
+			if (wave_any(U[0] < p.rhoMin)) W.P = U[0] < p.rhoMin ? R(0) : W.P;
 			return W;
 		}
 
@@ -232,7 +232,7 @@ struct EulerT {
 			// a constant index (a branch-merged store with a computed index
 			// would move Y to scratch memory)
 			const bool vac = e.rho < p.rhoMin;
-			const bool anyVac = __any(vac);
+			const bool anyVac = wave_any(vac);
 			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
 			const R Cs = e.Cs, vSq = e.vSq, g1 = p.g1;
 			R denom = R(2) * Cs * Cs;
@@ -277,11 +277,31 @@ struct EulerT {
 		template <int S>
 		__device__ static __forceinline__ void right(const Par &p, const Eig &e, const R *X, R *Y) {
 			const bool vac = e.rho < p.rhoMin;   // identity: selected per element, as in left
-			const bool anyVac = __any(vac);
+			const bool anyVac = wave_any(vac);
 			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
 			const R Cs = e.Cs;
 			R z[5];
 			z[0] = X[0] + X[1] + X[4];
+			if constexpr (FAST) {
+				// the same sums factored: z_n = vn z0 + c (X4 - X0), z_t = v_t z0 + X_t,
+				// z4 = H (X0 + X4) + c vn (X4 - X0) + vSq/2 X1 + v_t . X_t
+				const R vn = e.v[A], d40 = X[4] - X[0];
+				z[1 + A] = vn * z[0] + Cs * d40;
+				z[1 + B] = B < NVEL ? e.v[B] * z[0] + X[2] : R(0);
+				z[1 + C] = C < NVEL ? e.v[C] * z[0] + X[3] : R(0);
+				R y4 = e.hTotal * (X[0] + X[4]) + (Cs * vn) * d40 + (R(.5) * e.vSq) * X[1];
+				if constexpr (B < NVEL) y4 = y4 + e.v[B] * X[2];
+				if constexpr (C < NVEL) y4 = y4 + e.v[C] * X[3];
+				if (anyVac) {
+#pragma unroll
+					for (int i = 0; i < 5; ++i) Y[i] = vac ? X[i] : (i == 4 ? y4 : z[i]);
+				} else {
+#pragma unroll
+					for (int i = 0; i < 4; ++i) Y[i] = z[i];
+					Y[4] = y4;
+				}
+				return;
+			}
 			z[1 + A] = X[0] * (e.v[A] - Cs) + X[1] * e.v[A] + X[4] * (e.v[A] + Cs);
 			if constexpr (B < NVEL) z[1 + B] = X[0] * e.v[B] + X[1] * e.v[B] + X[2] + X[4] * e.v[B];
 			else z[1 + B] = R(0);

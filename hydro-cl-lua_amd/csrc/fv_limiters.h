@@ -70,6 +70,19 @@ __device__ __forceinline__ R limited(int index, R num, R den) {
 		// minmod(a, b): the operand nearer 0 when both have one sign, else 0,
 		// without compares: max(min(a, b), 0) + min(max(a, b), 0)
 		return fmax(fmin(num, den), R(0)) + fmin(fmax(num, den), R(0));
+	} else if constexpr (FAST && LIM == HYDRO_LIMITER_VAN_LEER) {
+		// (num |den| + |num| den) / (|num| + |den|) without sign tests: with
+		// one sign it is 2ab / (a + b) carrying den's sign (the two products
+		// are one rounded value each, so the sum doubles it exactly, as 2a * b
+		// does); with opposite signs the products cancel to +0. The products
+		// are separate statements so that neither is fused into the sum. Both
+		// 0, or a NaN: |num| + |den| is not positive and the result is 0.
+		const R p1 = num * fabs(den);
+		const R p2 = fabs(num) * den;
+		const R n = p1 + p2;
+		const R d = fabs(num) + fabs(den);
+		const R q = M::div(n, d);
+		return d > R(0) ? q : R(0);
 	} else if constexpr (FAST && limited_symmetric(LIM)) {
 		const bool same = (num > R(0) && den > R(0)) || (num < R(0) && den < R(0));
 		const R a = fabs(num), b = fabs(den);

@@ -17,6 +17,12 @@
 
 namespace hydro {
 
+// Wave-wide votes over the active lanes, from the lane predicate's own
+// compare mask (HIP's __all / __any first turn the predicate into an integer
+// and compare it again).
+__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
 template <typename R, bool FAST>
 struct Math;
 
@@ -56,7 +62,7 @@ struct Math<double, true> {
 		double q = div_pos(x, y);
 		double ay = __builtin_fabs(y);
 		bool ok = ay > 0x1p-900 && ay < 0x1p+900;
-		if (!__all(ok)) q = ok ? q : x / y;
+		if (!wave_all(ok)) q = ok ? q : x / y;
 		return q;
 	}
 	// positive, normal arguments (no guard: values outside that range are
@@ -92,7 +98,7 @@ struct Math<double, true> {
 		double d = __builtin_fma(-s, s, x);
 		double out = __builtin_fma(d, h, s);
 		bool ok = x > 0x1p-900 && x < 0x1p+900;
-		if (!__all(ok)) out = ok ? out : __builtin_sqrt(x);
+		if (!wave_all(ok)) out = ok ? out : __builtin_sqrt(x);
 		return out;
 	}
 };
@@ -116,7 +122,7 @@ struct Math<float, true> {
 		float q = x * __builtin_amdgcn_rcpf(y);
 		float ay = __builtin_fabsf(y);
 		bool ok = ay > 0x1p-100f && ay < 0x1p+100f;
-		if (!__all(ok)) q = ok ? q : x / y;
+		if (!wave_all(ok)) q = ok ? q : x / y;
 		return q;
 	}
 	__device__ static __forceinline__ float div_inv(float x, float, float inv) { return x * inv; }

@@ -602,7 +602,7 @@ __device__ __forceinline__ void iface_solve(const typename E::Par &p, const Cell
 			// HLL where sStar is not a number (and the NaN case): a wave-uniform
 			// branch, as no lane of a finite state takes it
 			const bool hllCase = !(useFL || useL || useR || useFR);
-			if (__any(hllCase)) {
+			if (wave_any(hllCase)) {
 				const R dLR = sR - sL, iLR = M::rcp(dLR);
 #pragma unroll
 				for (int j = 0; j < E::Q; ++j) {
@@ -859,7 +859,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			plain = plain && mxb[k].fixedFace < 0 && !mxb[k].mirror;
 		}
 		C cb[4];
-		if (__all(plain)) {
+		if (wave_all(plain)) {
 #pragma unroll
 			for (int k = 0; k < 4; ++k) load_plain<R, E>(g, src, (long)myb.idx * g.step1 + mxb[k].idx, cb[k]);
 		} else {
@@ -886,7 +886,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	I1 = Iface<R, E>{};
 
 	// the wave's x coordinates need no fixed values or mirror flips
-	const bool xPlain = __all(mx.fixedFace < 0 && !mx.mirror);
+	const bool xPlain = wave_all(mx.fixedFace < 0 && !mx.mirror);
 	auto loadRow = [&](int r, C &c) {
 		Mapped my = DIM >= 2 ? map_coord(r, ny, g.bc[2], g.bc[3], 2) : myIn;
 		if (xPlain && my.fixedFace < 0 && !my.mirror) load_plain<R, E>(g, src, (long)my.idx * g.step1 + mx.idx, c);
@@ -962,7 +962,7 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			if constexpr (RM == ROE_FLAGGED) {
 				// an irregular interface of this row: the redo kernel takes the
 				// strip from this row on; nothing of it is stored here
-				if (!__all(okYprev && okY && okX)) {
+				if (!wave_all(okYprev && okY && okX)) {
 					if (lane == 0) {
 						const unsigned k = atomicAdd(&g.dev->redoCount, 1u);
 						g.redo[k] = make_int2(sy * g.stripsX + sx, r);

@@ -4,20 +4,22 @@
 // multAdd... -> constrainU, and the new state's dt on the last stage;
 // fvsolver.lua:225-302, rk.lua:152-183, calcDT.cl:38-73), with a third axis.
 //
-// Work decomposition: a block of W waves owns a 64-column strip (x) of W
-// consecutive rows (y) and marches a chunk of at most 16 planes in z. In
-// every wave:
+// Work decomposition: a block owns a 64-column strip (x) of W consecutive
+// rows (y) and marches a chunk of at most 16 planes in z. It has W row waves,
+// one per row, and one halo wave. In every row wave:
 //   x neighbours: DPP lane shifts, and the strip's edge interfaces of all the
 //     chunk's planes computed in one pass before the planes (as in 2D);
 //   z neighbours: a sliding register window of planes (as y in 2D);
 //   y neighbours: the block's rows of the current plane, exchanged through
-//     LDS. Each interface in y is computed once per block: wave w takes the
-//     interface below its row; the interfaces outside the block (one above,
-//     and for the flux limiter the characteristic jumps one further out on
-//     each side) are taken by the edge waves. Waves that wait at the barriers
-//     leave their SIMD to waves of other blocks.
-// Three barriers per plane; every wave of a block runs the same planes, so
-// all of them reach every barrier.
+//     LDS; wave w computes the interface below its row.
+// The halo wave loads the rows outside the block (one plane ahead) and
+// computes the block's y interfaces that no row owns: the top one, and the
+// reconstruction edge (PLM) or characteristic jump (flux limiter) of the
+// interface below the bottom row. So every row wave does the same work in
+// each of the three barrier phases of a plane (with the z step in the phase
+// where the halo wave computes its interfaces), and a plane takes as long as
+// one row's work. Every wave of a block runs the same planes, so all of them
+// reach every barrier.
 #pragma once
 
 #include "fv_stage.h"
@@ -61,7 +63,7 @@ template <typename R, class E, int W>
 struct Lds3 {
 	static constexpr int NV = Cell<R, E>::NV;
 	R u[W + 4][NV][WAVE];      // U of rows y0-2 .. y0+W+1
-	R e[W + 3][NV][WAVE];      // PLM: R edges of rows y0-1 .. y0+W-1; flux limiter: dW at y0-3/2 .. y0+W+1/2
+	R e[W + 2][NV][WAVE];      // PLM: R edges of rows y0-1 .. y0+W-1; flux limiter: dW at y0-3/2 .. y0+W-1/2
 	R f[W + 1][E::Q][WAVE];    // area-weighted y fluxes at y0-1/2 .. y0+W-1/2
 };
 
@@ -76,8 +78,16 @@ __device__ __forceinline__ void lds_get(const R (&s)[NV][WAVE], R *v, int lane) 
 	for (int j = 0; j < NV; ++j) v[j] = s[j][lane];
 }
 
+// the chunk's output planes z0 .. z1-1 of block item bz
+template <typename R>
+__device__ __forceinline__ void chunk3d(const GridDesc<R> &g, int bz, int &z0, int &z1) {
+	z0 = bz * g.rowsPerWave;
+	z1 = min(z0 + g.rowsPerWave, g.n[2]);
+}
+
+// Row wave w of the block: row y0 + w of the strip through the chunk.
 template <typename R, template <typename> class EQ, bool PLM, bool LIM, int FLIM, int SLIM, int W>
-__device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const GridDesc<R> &g, int sx, int by, int bz,
+__device__ __forceinline__ double stage_rows3d(const StageArgs<R> &a, const GridDesc<R> &g, int sx, int by, int bz,
 	int w, double dt, int lane, Lds3<R, EQ<R>, W> &sh, XEdge<R, EQ<R>, PLM, LIM, EDGE_PLANES> &edge)
 {
 	using E = EQ<R>;
@@ -86,7 +96,6 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	constexpr int NV = C::NV;
 	static_assert(!E::SOURCE, "3D is built for the Euler equations");
 	static_assert(E::NW == NV, "dW shares the LDS slots of the states");
-	static_assert(W >= 2, "the edge waves of a block are distinct");
 	const typename E::Par par = E::par(a);
 	const int lim = PLM ? 0 : a.limiter;
 	const int slim = a.limiter;
@@ -106,16 +115,11 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	const R dt_dx0 = rdt / g.dx[0];
 	const R dt_dx1 = rdt / g.dx[1];
 	const R dt_dx2 = rdt / g.dx[2];
-	const int z0 = bz * g.rowsPerWave;
-	const int z1 = min(z0 + g.rowsPerWave, nz);
+	int z0, z1;
+	chunk3d(g, bz, z0, z1);
 
-	auto loadAt = [&](int yy, int zz, C &c) {
-		Mapped my = map_coord(yy, ny, g.bc[2], g.bc[3], 2);
-		Mapped mz = map_coord(zz, nz, g.bc[4], g.bc[5], 4);
-		load_cell3<R, E>(g, src, mx, my, mz, c);
-	};
 	// the wave's x coordinates and row need no fixed values or mirror flips
-	const bool xyPlain = __all(mx.fixedFace < 0 && !mx.mirror) && myOwn.fixedFace < 0 && !myOwn.mirror;
+	const bool xyPlain = wave_all(mx.fixedFace < 0 && !mx.mirror) && myOwn.fixedFace < 0 && !myOwn.mirror;
 	auto loadPlane = [&](int zz, C &c) {
 		Mapped mz = map_coord(zz, nz, g.bc[4], g.bc[5], 4);
 		if (xyPlain && mz.fixedFace < 0 && !mz.mirror)
@@ -139,7 +143,7 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 			plain = plain && mxb[k].fixedFace < 0 && !mxb[k].mirror;
 		}
 		C cb[4];
-		if (__all(plain)) {
+		if (wave_all(plain)) {
 #pragma unroll
 			for (int k = 0; k < 4; ++k)
 				load_plain<R, E>(g, src, ((long)mz.idx * g.n[1] + myOwn.idx) * g.step1 + mxb[k].idx, cb[k]);
@@ -169,11 +173,6 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	if constexpr (PLM) { loadPlane(z0 - 2, W0); loadPlane(z0 - 1, W1); }
 	else if constexpr (LIM) { loadPlane(z0 - 2, W1); }
 	else { loadPlane(z0 - 1, W0); }
-	C H0, H1;
-	auto loadHalo = [&](int zz) {
-		if (w == 0) { loadAt(y0 - 2, zz, H0); loadAt(y0 - 1, zz, H1); }
-		if (w == W - 1) { loadAt(y0 + W, zz, H0); loadAt(y0 + W + 1, zz, H1); }
-	};
 
 	// two passes: the warm-up planes (z pipeline only), then the output
 	// planes, so that the output planes' body is one block with all the work
@@ -182,6 +181,12 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	const int rBegin = pass == 0 ? z0 - WARM : z0;
 	const int rEnd = pass == 0 ? z0 : z1;
 	for (int r = rBegin; r < rEnd; ++r) {
+		// plane r of this wave's row (the window's first plane) to the block
+		if (pass == 1) {
+			lds_put<R, E, NV>(sh.u[w + 2], W0.v, lane);
+			__syncthreads();
+		}
+
 		// ---- z direction: interface r+1/2, dzterm of plane r
 		R dzterm[Q];
 		{
@@ -226,49 +231,18 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 		if (pass == 1) {
 			const C &U = W0;     // plane r of this wave's row
 
-			// ---- y direction through LDS
-			lds_put<R, E, NV>(sh.u[w + 2], U.v, lane);
-			loadHalo(r);
-			if (w == 0) {
-				lds_put<R, E, NV>(sh.u[0], H0.v, lane);
-				lds_put<R, E, NV>(sh.u[1], H1.v, lane);
-			}
-			if (w == W - 1) {
-				lds_put<R, E, NV>(sh.u[W + 2], H0.v, lane);
-				lds_put<R, E, NV>(sh.u[W + 3], H1.v, lane);
-			}
-			__syncthreads();
-
-			R Fy[Q], FyTop[Q];
+			// ---- y direction through LDS: the interface below the row
+			R Fy[Q];
 			if constexpr (PLM) {
 				C Um, Up, Ly, Ry;
 				lds_get<R, E, NV>(sh.u[w + 1], Um.v, lane);
 				lds_get<R, E, NV>(sh.u[w + 3], Up.v, lane);
 				plm_cons<R, E, SLIM, 1>(par, slim, Um, U, Up, Ly, Ry, dt_dx1);
 				lds_put<R, E, NV>(sh.e[w + 1], Ry.v, lane);
-				C Ltop;
-				if (w == 0) {
-					C a0, a1, a2, Le, Re;
-					lds_get<R, E, NV>(sh.u[0], a0.v, lane);
-					lds_get<R, E, NV>(sh.u[1], a1.v, lane);
-					lds_get<R, E, NV>(sh.u[2], a2.v, lane);
-					plm_cons<R, E, SLIM, 1>(par, slim, a0, a1, a2, Le, Re, dt_dx1);
-					lds_put<R, E, NV>(sh.e[0], Re.v, lane);
-				}
-				if (w == W - 1) {
-					C a0, a1, a2, Re;
-					lds_get<R, E, NV>(sh.u[W + 1], a0.v, lane);
-					lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
-					lds_get<R, E, NV>(sh.u[W + 3], a2.v, lane);
-					plm_cons<R, E, SLIM, 1>(par, slim, a0, a1, a2, Ltop, Re, dt_dx1);
-				}
 				__syncthreads();
 				C Rm;
 				lds_get<R, E, NV>(sh.e[w], Rm.v, lane);
 				iface_solve<R, E, 1>(par, Rm, Ly, dt_dx1, lim, Fy);
-				if (w == W - 1) {
-					iface_solve<R, E, 1>(par, Ry, Ltop, dt_dx1, lim, FyTop);
-				}
 			} else if constexpr (LIM) {
 				// slot k of sh.e holds dW of interface y0 + k - 3/2
 				C Um;
@@ -276,54 +250,22 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 				Iface<R, E> I;
 				iface_prep<R, E, 1>(par, Um, U, I);
 				lds_put<R, E, NV>(sh.e[w + 1], I.dW, lane);
-				Iface<R, E> It;
-				if (w == 0) {
-					C a0, a1;
-					lds_get<R, E, NV>(sh.u[0], a0.v, lane);
-					lds_get<R, E, NV>(sh.u[1], a1.v, lane);
-					Iface<R, E> Ie;
-					iface_prep<R, E, 1>(par, a0, a1, Ie);
-					lds_put<R, E, NV>(sh.e[0], Ie.dW, lane);
-				}
-				if (w == W - 2) {
-					C a0, a1;
-					lds_get<R, E, NV>(sh.u[W + 2], a0.v, lane);
-					lds_get<R, E, NV>(sh.u[W + 3], a1.v, lane);
-					Iface<R, E> Ie;
-					iface_prep<R, E, 1>(par, a0, a1, Ie);
-					lds_put<R, E, NV>(sh.e[W + 2], Ie.dW, lane);
-				}
-				if (w == W - 1) {
-					C a1;
-					lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
-					iface_prep<R, E, 1>(par, U, a1, It);
-					lds_put<R, E, NV>(sh.e[W + 1], It.dW, lane);
-				}
 				__syncthreads();
 				R dWL[E::NW], dWR[E::NW];
 				lds_get<R, E, NV>(sh.e[w], dWL, lane);
 				lds_get<R, E, NV>(sh.e[w + 2], dWR, lane);
 				iface_flux<R, E, 1, true, FLIM>(par, I, dWL, dWR, dt_dx1, lim, Fy);
-				if (w == W - 1) {
-					lds_get<R, E, NV>(sh.e[W + 2], dWR, lane);
-					iface_flux<R, E, 1, true, FLIM>(par, It, I.dW, dWR, dt_dx1, lim, FyTop);
-				}
 			} else {
 				C Um;
 				lds_get<R, E, NV>(sh.u[w + 1], Um.v, lane);
 				iface_solve<R, E, 1>(par, Um, U, dt_dx1, lim, Fy);
-				if (w == W - 1) {
-					C a1;
-					lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
-					iface_solve<R, E, 1>(par, U, a1, dt_dx1, lim, FyTop);
-				}
+				__syncthreads();
 			}
 			R FaY[Q];
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
 				FaY[j] = flux_on<E>(g, 1) ? Fy[j] * flux_aov<E>(g, 1) : R(0);
 				sh.f[w][j][lane] = FaY[j];
-				if (w == W - 1) sh.f[W][j][lane] = flux_on<E>(g, 1) ? FyTop[j] * flux_aov<E>(g, 1) : R(0);
 			}
 			__syncthreads();
 			R dyterm[Q];
@@ -405,9 +347,106 @@ __device__ __forceinline__ double stage_block3d(const StageArgs<R> &a, const Gri
 	return dtLocal;
 }
 
-// One block of W waves per (strip, row block, plane chunk); grid = items.
+// The halo wave of the block: per output plane, the rows y0-2, y0-1 and
+// y0+W, y0+W+1 to LDS (loaded one plane ahead), then the y interface
+// quantities no row wave owns: the R edge of row y0-1 (PLM) or dW of
+// interface y0-3/2 (flux limiter) for row wave 0, and the flux of the top
+// interface y0+W-1/2. The same barriers as the row waves.
+template <typename R, template <typename> class EQ, bool PLM, bool LIM, int FLIM, int SLIM, int W>
+__device__ __forceinline__ void stage_halo3d(const StageArgs<R> &a, const GridDesc<R> &g, int sx, int by, int bz,
+	double dt, int lane, Lds3<R, EQ<R>, W> &sh)
+{
+	using E = EQ<R>;
+	using C = Cell<R, E>;
+	constexpr int Q = E::Q;
+	constexpr int NV = C::NV;
+	constexpr bool TWO = PLM || LIM;    // two halo rows on each side, else one
+	const typename E::Par par = E::par(a);
+	const int lim = PLM ? 0 : a.limiter;
+	const int slim = a.limiter;
+	const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+	const R *src = g.U[a.src];
+	const int x = sx * STRIP_OUT + lane;
+	const int y0 = by * W;
+	const Mapped mx = map_coord(x, nx, g.bc[0], g.bc[1], 0);
+	const R dt_dx1 = R(dt) / g.dx[1];
+	int z0, z1;
+	chunk3d(g, bz, z0, z1);
+
+	// halo row k: rows y0-2, y0-1, y0+W, y0+W+1 (one halo row per side:
+	// y0-1 and y0+W in slots 1 and 2)
+	const int rowOf[4] = {y0 - 2, y0 - 1, y0 + W, y0 + W + 1};
+	const int slotOf[4] = {0, 1, W + 2, W + 3};
+	C h[4];
+	auto loadHalo = [&](int zz) {
+		const Mapped mz = map_coord(zz, nz, g.bc[4], g.bc[5], 4);
+#pragma unroll
+		for (int k = 0; k < 4; ++k) {
+			if (!TWO && (k == 0 || k == 3)) continue;
+			const Mapped my = map_coord(rowOf[k], ny, g.bc[2], g.bc[3], 2);
+			load_cell3<R, E>(g, src, mx, my, mz, h[k]);
+		}
+	};
+	if (z0 < z1) loadHalo(z0);
+	for (int r = z0; r < z1; ++r) {
+#pragma unroll
+		for (int k = 0; k < 4; ++k) {
+			if (!TWO && (k == 0 || k == 3)) continue;
+			lds_put<R, E, NV>(sh.u[slotOf[k]], h[k].v, lane);
+		}
+		if (r + 1 < z1) loadHalo(r + 1);
+		__syncthreads();
+
+		R F[Q];
+		if constexpr (PLM) {
+			C a0, a1, a2, Le, Re, Ltop;
+			lds_get<R, E, NV>(sh.u[0], a0.v, lane);
+			lds_get<R, E, NV>(sh.u[1], a1.v, lane);
+			lds_get<R, E, NV>(sh.u[2], a2.v, lane);
+			plm_cons<R, E, SLIM, 1>(par, slim, a0, a1, a2, Le, Re, dt_dx1);
+			lds_put<R, E, NV>(sh.e[0], Re.v, lane);
+			lds_get<R, E, NV>(sh.u[W + 1], a0.v, lane);
+			lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
+			lds_get<R, E, NV>(sh.u[W + 3], a2.v, lane);
+			plm_cons<R, E, SLIM, 1>(par, slim, a0, a1, a2, Ltop, Re, dt_dx1);
+			__syncthreads();
+			C Rtop;
+			lds_get<R, E, NV>(sh.e[W], Rtop.v, lane);     // row y0+W-1's R edge
+			iface_solve<R, E, 1>(par, Rtop, Ltop, dt_dx1, lim, F);
+		} else if constexpr (LIM) {
+			C a0, a1;
+			Iface<R, E> Ie, It, Iu;
+			lds_get<R, E, NV>(sh.u[0], a0.v, lane);
+			lds_get<R, E, NV>(sh.u[1], a1.v, lane);
+			iface_prep<R, E, 1>(par, a0, a1, Ie);              // interface y0-3/2
+			lds_put<R, E, NV>(sh.e[0], Ie.dW, lane);
+			lds_get<R, E, NV>(sh.u[W + 1], a0.v, lane);
+			lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
+			iface_prep<R, E, 1>(par, a0, a1, It);              // interface y0+W-1/2
+			lds_put<R, E, NV>(sh.e[W + 1], It.dW, lane);
+			lds_get<R, E, NV>(sh.u[W + 3], a0.v, lane);
+			iface_prep<R, E, 1>(par, a1, a0, Iu);              // interface y0+W+1/2
+			__syncthreads();
+			R dWL[E::NW];
+			lds_get<R, E, NV>(sh.e[W], dWL, lane);            // interface y0+W-3/2
+			iface_flux<R, E, 1, true, FLIM>(par, It, dWL, Iu.dW, dt_dx1, lim, F);
+		} else {
+			C a0, a1;
+			lds_get<R, E, NV>(sh.u[W + 1], a0.v, lane);
+			lds_get<R, E, NV>(sh.u[W + 2], a1.v, lane);
+			iface_solve<R, E, 1>(par, a0, a1, dt_dx1, lim, F);
+			__syncthreads();
+		}
+#pragma unroll
+		for (int j = 0; j < Q; ++j) sh.f[W][j][lane] = flux_on<E>(g, 1) ? F[j] * flux_aov<E>(g, 1) : R(0);
+		__syncthreads();
+	}
+}
+
+// One block of Block3::WAVES waves per (strip, row block, plane chunk): the
+// row waves 0 .. W-1 and the halo wave W; grid = items.
 template <typename R, template <typename> class EQ, bool PLM, bool LIM, int FLIM, int SLIM>
-__global__ void __launch_bounds__(WAVE * Block3<R>::W, 3) stage_kernel3d(const StageArgs<R> a)
+__global__ void __launch_bounds__(WAVE * Block3<R>::WAVES, 3) stage_kernel3d(const StageArgs<R> a)
 {
 	constexpr int W = Block3<R>::W;
 	__shared__ Lds3<R, EQ<R>, W> sh;
@@ -422,7 +461,11 @@ __global__ void __launch_bounds__(WAVE * Block3<R>::W, 3) stage_kernel3d(const S
 	const int by = rest % g.stripsY;
 	const int bz = rest / g.stripsY;
 	if (blk == 0 && w == 0) stage_first_strip(a, g, dt, lane);
-	double dtLocal = stage_block3d<R, EQ, PLM, LIM, FLIM, SLIM, W>(a, g, sx, by, bz, w, dt, lane, sh, edges[w]);
+	if (w == W) {
+		stage_halo3d<R, EQ, PLM, LIM, FLIM, SLIM, W>(a, g, sx, by, bz, dt, lane, sh);
+		return;
+	}
+	double dtLocal = stage_rows3d<R, EQ, PLM, LIM, FLIM, SLIM, W>(a, g, sx, by, bz, w, dt, lane, sh, edges[w]);
 	stage_reduce(a, g, dtLocal, (long)blk * W + w, lane);
 }
 

@@ -32,13 +32,15 @@ constexpr int EDGE_PLANES = 16;    // planes per block chunk at most (3D)
 // 3D stage kernel: a block is W row waves (rows of a strip that share one
 // plane through LDS) and one halo wave that loads the rows outside the block
 // and computes the y interfaces no row owns, so that the row waves do equal
-// work between barriers. Four waves, so that three blocks fit per CU at the
-// kernel's register use and one block's barrier waits are covered by the
-// others' work.
+// work between barriers. Four waves, so that several blocks fit per CU and
+// one block's barrier waits are covered by the others' work. MIN_BLOCKS
+// bounds the registers (FP32: three blocks per CU, at most 168 VGPRs; FP64
+// takes what it needs: capped, its kernels spill).
 template <typename R>
 struct Block3 {
 	static constexpr int W = 3;
 	static constexpr int WAVES = W + 1;
+	static constexpr int MIN_BLOCKS = sizeof(R) == 4 ? 3 : 1;
 };
 
 // Device-resident per-solver scalars: dt minima rotate through three slots so

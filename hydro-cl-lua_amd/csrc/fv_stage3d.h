@@ -446,7 +446,7 @@ __device__ __forceinline__ void stage_halo3d(const StageArgs<R> &a, const GridDe
 // One block of Block3::WAVES waves per (strip, row block, plane chunk): the
 // row waves 0 .. W-1 and the halo wave W; grid = items.
 template <typename R, template <typename> class EQ, bool PLM, bool LIM, int FLIM, int SLIM>
-__global__ void __launch_bounds__(WAVE * Block3<R>::WAVES, 3) stage_kernel3d(const StageArgs<R> a)
+__global__ void __launch_bounds__(WAVE * Block3<R>::WAVES, Block3<R>::MIN_BLOCKS) stage_kernel3d(const StageArgs<R> a)
 {
 	constexpr int W = Block3<R>::W;
 	__shared__ Lds3<R, EQ<R>, W> sh;

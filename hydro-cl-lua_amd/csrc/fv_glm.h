@@ -69,6 +69,24 @@ struct Eqn {
 	// eigen_leftTransform (glm-maxwell.cl:175-252)
 	template <int S>
 	__device__ static __forceinline__ void left(const Par &, const Eig &g, const R *X, R *Y) {
+		if constexpr (FAST) {
+			// the same sums without their zero terms: with the normal along
+			// axis A and the tangents B, C of coord.lua's normal_l
+			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
+			const R e = g.e, m = g.m;
+			const R h = M::rcp(e) * R(.5);
+			const R mB = X[3 + B] * m, mC = X[3 + C] * m;
+			const R eB = X[B] * e, eC = X[C] * e;
+			Y[0] = h * (X[6] - X[A]);
+			Y[1] = h * (X[7] - X[3 + A]);
+			Y[2] = R(.5) * (mB + eC);
+			Y[3] = R(.5) * (mC - eB);
+			Y[4] = R(.5) * (mB - eC);
+			Y[5] = R(.5) * (mC + eB);
+			Y[6] = h * (X[6] + X[A]);
+			Y[7] = h * (X[7] + X[3 + A]);
+			return;
+		}
 		const R n_l[3] = {nl<S, 1, 1>(), nl<S, 1, 2>(), nl<S, 1, 3>()};
 		const R n2_l[3] = {nl<S, 2, 1>(), nl<S, 2, 2>(), nl<S, 2, 3>()};
 		const R n3_l[3] = {nl<S, 3, 1>(), nl<S, 3, 2>(), nl<S, 3, 3>()};
@@ -143,6 +161,20 @@ struct Eqn {
 	// eigen_rightTransform (glm-maxwell.cl:257-298)
 	template <int S>
 	__device__ static __forceinline__ void right(const Par &, const Eig &g, const R *X, R *Y) {
+		if constexpr (FAST) {
+			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
+			const R e = g.e;
+			const R ie = M::rcp(e), im = M::rcp(g.m);
+			Y[A] = e * (X[6] - X[0]);
+			Y[B] = ie * (X[5] - X[3]);
+			Y[C] = ie * (X[2] - X[4]);
+			Y[3 + A] = e * (X[7] - X[1]);
+			Y[3 + B] = im * (X[4] + X[2]);
+			Y[3 + C] = im * (X[5] + X[3]);
+			Y[6] = e * (X[6] + X[0]);
+			Y[7] = e * (X[7] + X[1]);
+			return;
+		}
 		const R n_l[3] = {nl<S, 1, 1>(), nl<S, 1, 2>(), nl<S, 1, 3>()};
 		const R n2_l[3] = {nl<S, 2, 1>(), nl<S, 2, 2>(), nl<S, 2, 3>()};
 		const R n3_l[3] = {nl<S, 3, 1>(), nl<S, 3, 2>(), nl<S, 3, 3>()};
@@ -178,6 +210,19 @@ struct Eqn {
 	template <int S>
 	__device__ static __forceinline__ void flux(const Par &p, const R *U, const Prim &, R *F) {
 		R a2 = U[10] * U[10], b2 = U[11] * U[11];
+		if constexpr (FAST) {
+			// F_D = H x n + n phi divPhi, F_B = n psi divPsi - E x n, n along A
+			constexpr int A = S, B = (S + 1) % 3, C = (S + 2) % 3;
+			F[A] = U[6] * p.divPhi;
+			F[B] = U[3 + C] * b2;
+			F[C] = -(U[3 + B] * b2);
+			F[3 + A] = U[7] * p.divPsi;
+			F[3 + B] = -(U[C] * a2);
+			F[3 + C] = U[B] * a2;
+			F[6] = U[S] * p.divPhi;
+			F[7] = U[3 + S] * p.divPsi;
+			return;
+		}
 		R E0 = U[0] * a2, E1 = U[1] * a2, E2 = U[2] * a2;
 		R H0 = U[3] * b2, H1 = U[4] * b2, H2 = U[5] * b2;
 		const R nx = nl<S, 1, 1>(), ny = nl<S, 1, 2>(), nz = nl<S, 1, 3>();
@@ -223,6 +268,22 @@ struct Eqn {
 		for (int j = 0; j < DIM; ++j) g[j] = (ep[j] * ep[j] - em[j] * em[j]) * invdx[j];
 		Prim W;
 		R F[8];
+		if constexpr (FAST) {
+			// the same sums without the zero gradients (axes beyond DIM, and
+			// the B rows' zero grad_1_eps)
+#pragma unroll
+			for (int i = 0; i < DIM; ++i) {
+				if (i == 0) flux<0>(p, U, W, F);
+				if (i == 1) flux<1>(p, U, W, F);
+				if (i == 2) flux<2>(p, U, W, F);
+				R s = F[0] * g[0];
+				if constexpr (DIM > 1) s = s + F[1] * g[1];
+				if constexpr (DIM > 2) s = s + F[2] * g[2];
+				d[i] = d[i] - s;
+			}
+			d[6] = d[6] + U[8] * p.divPhi;
+			return;
+		}
 		flux<0>(p, U, W, F);
 		d[0] = d[0] - (F[0] * g[0] + F[1] * g[1] + F[2] * g[2]);
 		d[3] = d[3] - (F[3] * z[0] + F[4] * z[1] + F[5] * z[2]);

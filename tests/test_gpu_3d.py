@@ -2,8 +2,9 @@
 
 FP64 (IEEE and fast) within 1e-12 relative L1 per conserved variable, FP32
 bulk within 1e-5 of the FP64 oracle. The grids are small and uneven so that
-strips, row blocks (4 rows) and plane chunks end inside the grid,
-and every boundary kind meets every face.
+strips, row blocks (3 rows and the halo wave) and plane chunks end inside the
+grid, and every boundary kind meets every face; grids of 1 and 2 rows put
+row waves outside the grid and the halo wave's rows through the y faces.
 """
 import numpy as np
 import pytest
@@ -102,6 +103,27 @@ def test_3d_fixed_faces_and_forward_euler():
     err, s, o = _run(_octants(3), 8, size=(18, 10, 12), boundary=bc, boundary_fixed=fixed,
                      flux_limiter='minmod', integrator='forward Euler', cfl=.3)
     assert np.all(err <= TOL64), err
+
+
+@pytest.mark.parametrize('size,mode,bc,fast', [
+    ((20, 1, 10), 'superbee', 'periodic', False),
+    ((17, 2, 9), 'plm', 'mirror', False),
+    ((19, 2, 7), 'none', 'freeflow', False),
+    ((13, 1, 8), 'van Leer', 'mirror', True),
+    ((22, 4, 9), 'van Leer', {'xmin': 'periodic', 'xmax': 'periodic', 'ymin': 'mirror', 'ymax': 'freeflow',
+                              'zmin': 'freeflow', 'zmax': 'freeflow'}, True),
+])
+def test_3d_thin_in_y(size, mode, bc, fast):
+    """fewer rows than a block (or one more): the rows outside the grid are
+    computed, never stored, and the halo rows come through the y faces"""
+    kw = dict(size=size, boundary=bc, integrator='Runge-Kutta 2, TVD', cfl=.3)
+    if mode == 'plm':
+        kw.update(plm='plm cons', slope_limiter='minmod')
+    elif mode != 'none':
+        kw.update(flux_limiter=mode)
+    err, s, o = _run(_octants(7), 6, fast=fast, **kw)
+    assert np.all(err <= TOL64), (size, mode, err)
+    assert s.t == o.t or fast
 
 
 @pytest.mark.parametrize("rows", [4, 7, 16])

@@ -376,6 +376,22 @@ def serving_leg(count, seed=0, steps_per_tick=20, budget=32 << 30, closed_only=F
     lat, mk, _ = serve(reqs_p, budget, steps_per_tick)
     lat_s, mk_s, _ = serve_sequential(reqs, closed_loop=True)
     kinds = {k: sum(r.kind == k for r in reqs) for k in ('euler1d', 'euler2d', 'glm2d')}
+    # each kind served alone, against its share of the sequential run (the
+    # sequential run serves the requests in order, so a request's time is the
+    # gap between its completion and the one before)
+    done = np.sort(np.asarray(lat_s))
+    took = np.diff(np.concatenate([[0.], done]))
+    took_of = dict(zip(np.argsort(np.asarray(lat_s), kind='stable'), took))
+    alone = {}
+    for k in kinds:
+        rk = [r for r in reqs if r.kind == k]
+        if not rk:
+            continue
+        _, mk_k, _ = serve(rk, budget, steps_per_tick, closed_loop=True)
+        seq_k = float(sum(took_of[i] for i, r in enumerate(reqs) if r.kind == k))
+        cu_k = sum(r.cell_updates() for r in rk)
+        alone[k] = {'served_makespan_s': mk_k, 'served_gcell_updates_per_s': cu_k / mk_k / 1e9,
+                    'sequential_s': seq_k, 'speedup_vs_sequential': seq_k / mk_k}
     by_kind = {}
     for k in kinds:
         idx = [i for i, r in enumerate(reqs_p) if r.kind == k]
@@ -387,7 +403,7 @@ def serving_leg(count, seed=0, steps_per_tick=20, budget=32 << 30, closed_only=F
         'workload': f'{count} solve requests (seed {seed}): 1D Euler N in [256, 65536], 2D Euler side '
                     f'log-uniform [64, 1024], 2D GLM-Maxwell side log-uniform [128, 512] (sigma 0, eps = mu = 1); '
                     f'100-2000 steps each, FP32, Roe + plm cons/minmod, RK2-TVD',
-        'mix': kinds, 'cell_updates': cu, 'poisson_by_kind': by_kind,
+        'mix': kinds, 'cell_updates': cu, 'poisson_by_kind': by_kind, 'closed_loop_alone_by_kind': alone,
         'closed_loop_solves_per_s': capacity, 'closed_loop_gcell_updates_per_s': cu / mk_c / 1e9,
         'poisson_rate_per_s': rate, 'solves_per_s': count / mk,
         'p50_latency_s': float(np.percentile(lat, 50)), 'p99_latency_s': float(np.percentile(lat, 99)),

@@ -125,6 +125,7 @@ struct StageArgs {
 	int reduceSlot;           // slot to min-reduce the new state's dt into, -1 none
 	int advanceTime;          // t += dt, dt stored
 	int redoPass;             // this launch is the redo kernel of the stage
+	int shareEdges;           // plain 2D launch: one strip-edge pass per block of 4 x-adjacent strips
 	int stepInTick;           // batch: skip grids with stepsLeft <= stepInTick
 	// equation parameters in the stencil type and in double (for the FP64
 	// accumulation, constrainU and dt work of the FP32 form)

@@ -251,12 +251,13 @@ static long device_cus() {
 // Rows per wave of the plain stage kernel (2D), from KH sweeps (rows 3..32 at
 // 512^2 .. 4096^2, fast math). Each wave marches `rows` rows of a 64-column
 // strip after a warm-up of 1-3 rows and its strip-edge pass (about 1.5 rows of
-// work). FP64 is VALU-bound at 3 resident waves per SIMD: blocks of 4 waves
-// are dealt to the CUs, a CU runs L = ceil(blocks / CUs) of them, and the
-// launch takes about L * (rows + warm + 1.5) / eff(L), eff(L) (fitted) being
-// how well L waves per SIMD hide latency; the rows in 4..32 with the least
-// time are taken (32 being the most rows whose strip edges one wave computes
-// in a single pass). FP32 is latency-bound and gains from many short waves:
+// work; about 0.5 when a block's four x-adjacent strips share one pass).
+// FP64 holds 4 waves per SIMD: blocks of 4 waves are dealt to the CUs, a CU
+// runs L = ceil(blocks / CUs) of them, and the launch takes about
+// L * (rows + warm + edge) / eff(L), eff(L) (fitted: .7, .77, .9, 1 for L = 1,
+// 2, 3, 4+) being how well L waves per SIMD hide latency; the rows in 4..8
+// with the least time are taken: 4 at 512^2 and 1024^2 (one round of 4 waves
+// per SIMD), 8 at 2048^2 and 4096^2. FP32 is latency-bound and gains from many short waves:
 // rows = clamp(strips * ny / 4096, 4, 8) (4 at 512^2 .. 1024^2, 8 from 2048^2;
 // longer waves leave a costly partial last round).
 // 3D: planes per block chunk (at most 16). Blocks are small (4 waves) and many resident
@@ -282,7 +283,7 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 	}
 	if (cfg.precision == HYDRO_REAL_F32)
 		return (int)std::max(4L, std::min(8L, (stripsX * (long)ny + 4095) / 4096));
-	static const double eff[4] = {0, .7, .9, 1.};
+	static const double eff[5] = {0, .7, .77, .9, 1.};
 	const long cus = device_cus();
 	int best = 4;
 	double bestT = -1;
@@ -290,7 +291,9 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 	for (int r = 4; r <= 8; ++r) {
 		long blocks = (stripsX * ((ny + r - 1) / r) + 3) / 4;
 		long L = (blocks + cus - 1) / cus;
-		double t = (double)L * (r + warm + 1.5) / eff[std::min(L, 3L)];
+		// the edge pass: a wave's own, or a quarter of its block's shared one
+		const double edge = stripsX % 4 == 0 && 5 * r <= WAVE ? .5 : 1.5;
+		double t = (double)L * (r + warm + edge) / eff[std::min(L, 4L)];
 		if (bestT < 0 || t < bestT * (1 - 1e-9)) { best = r; bestT = t; }
 	}
 	return best;
@@ -478,6 +481,7 @@ static void stage_args(const Grid &g, int s, StageArgs<R> &a) {
 	a.advanceTime = 0;
 	a.xcdRemap = 1;
 	a.redoPass = 0;
+	a.shareEdges = 0;
 }
 
 static StageKey stage_key(const Grid &g) {

@@ -720,24 +720,26 @@ __device__ __forceinline__ void edge_pass(const typename E::Par &par, int lim, i
 
 // x part of a row: the x flux divergence (area-weighted, -(F(x+1/2) - F(x-1/2)))
 // of row U, lanes exchanging neighbours by DPP and, with EDGES, the strip
-// edges (lanes 0 and 63) taken from the row's edge slot j (without, lanes 0-1
-// and 62-63 are a halo whose results are not used). Um / Up return the x
-// neighbours.
+// edges (lanes 0 and 63) taken from the edge slots slotL and slotR of edge
+// (the cell outside the right edge from edgeR's slot; a wave's own slice
+// holds both) (without, lanes 0-1 and 62-63 are a halo whose results are not
+// used). Um / Up return the x neighbours.
 template <typename R, class E, bool PLM, bool LIM, int FLIM, int SLIM, int NR, bool EDGES = true, int RM = ROE_INLINE>
 __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int slim, const Cell<R, E> &U, R dt_dx0,
-	bool fluxOn, R aov, int lane, int j, const XEdge<R, E, PLM, LIM, NR> &edge, Cell<R, E> &Um, Cell<R, E> &Up,
-	R *divx, bool &okx)
+	bool fluxOn, R aov, int lane, int slotL, int slotR, const XEdge<R, E, PLM, LIM, NR> &edge,
+	const XEdge<R, E, PLM, LIM, NR> &edgeR, Cell<R, E> &Um, Cell<R, E> &Up, R *divx, bool &okx)
 {
 	using C = Cell<R, E>;
 	constexpr int Q = E::Q;
 	okx = true;
-	const int slot = (lane == WAVE - 1 ? NR : 0) + j;
+	const int slot = lane == WAVE - 1 ? slotR : slotL;
 	if constexpr (EDGES && XEdge<R, E, PLM, LIM, NR>::U) {
 		// lane 0 reads the left edge slot, lane 63 the right one: the shifts'
 		// empty lanes take them
+		const XEdge<R, E, PLM, LIM, NR> &eu = lane == WAVE - 1 ? edgeR : edge;
 #pragma unroll
 		for (int f = 0; f < C::NV; ++f) {
-			R e = edge.u[f][slot];
+			R e = eu.u[f][slot];
 			Um.v[f] = from_left_or(U.v[f], e);
 			Up.v[f] = from_right_or(U.v[f], e);
 		}
@@ -815,7 +817,7 @@ __device__ __forceinline__ void strip_x(const typename E::Par &par, int lim, int
 template <typename R, template <typename> class EQ, int DIM, bool PLM, bool LIM, int FLIM, int SLIM,
 	int RM = ROE_INLINE>
 __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridDesc<R> &g, int sx, int sy, double dt,
-	int lane, XEdge<R, EQ<R>, PLM, LIM, EDGE_ROWS> &edge, int yStart = -1)
+	int lane, XEdge<R, EQ<R>, PLM, LIM, EDGE_ROWS> *edges, int w, bool share, int yStart = -1)
 {
 	using E = EQ<R>;
 	using C = Cell<R, E>;
@@ -844,30 +846,82 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 	const int y1 = min(sy * g.rowsPerWave + g.rowsPerWave, ny);
 
 	// ---- the strip's edge interfaces for rows y0 .. y1-1: lane j (< 32) the
-	// left edge of row y0 + j, lane 32 + j the right edge
+	// left edge of row y0 + j, lane 32 + j the right edge. Shared (plain 2D
+	// launch, a block's four waves x-adjacent strips of one row band): wave 0
+	// computes the block's five edge columns, lane c * rows + j the interface
+	// left of strip sx + c in row y0 + j, into slice 0 (the right cells of the
+	// interfaces into slice 1); one barrier publishes them.
+	using XE = XEdge<R, E, PLM, LIM, EDGE_ROWS>;
+	XE &edge = edges[share ? 0 : w];
+	const XE &edgeR = edges[share ? 1 : w];
+	const int rpw = g.rowsPerWave;
 	if constexpr (EDGES) {
-		const int bj = lane & (EDGE_ROWS - 1);
-		const bool right = lane >= EDGE_ROWS;
-		const int yb = y0 + (bj < y1 - y0 ? bj : 0);
-		const int xb = sx * STRIP_OUT + (right ? STRIP_OUT : 0);   // interface xb - 1/2
-		const Mapped myb = DIM >= 2 ? map_coord(yb, ny, g.bc[2], g.bc[3], 2) : myIn;
-		Mapped mxb[4];
-		bool plain = myb.fixedFace < 0 && !myb.mirror;
+		if (share) {
+			if (w == 0) {
+				int c = lane / rpw, j = lane - c * rpw;
+				if (c > 4) { c = 4; j = 0; }       // lanes past the five columns repeat one
+				const int yb = y0 + (j < y1 - y0 ? j : 0);
+				const int xb = (sx + c) * STRIP_OUT;   // interface xb - 1/2
+				const Mapped myb = map_coord(yb, ny, g.bc[2], g.bc[3], 2);
+				Mapped mxb[4];
+				bool plain = myb.fixedFace < 0 && !myb.mirror;
 #pragma unroll
-		for (int k = 0; k < 4; ++k) {
-			mxb[k] = map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0);
-			plain = plain && mxb[k].fixedFace < 0 && !mxb[k].mirror;
-		}
-		C cb[4];
-		if (wave_all(plain)) {
+				for (int k = 0; k < 4; ++k) {
+					mxb[k] = map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0);
+					plain = plain && mxb[k].fixedFace < 0 && !mxb[k].mirror;
+				}
+				C cb[4];
+				if (wave_all(plain)) {
 #pragma unroll
-			for (int k = 0; k < 4; ++k) load_plain<R, E>(g, src, (long)myb.idx * g.step1 + mxb[k].idx, cb[k]);
+					for (int k = 0; k < 4; ++k) load_plain<R, E>(g, src, (long)myb.idx * g.step1 + mxb[k].idx, cb[k]);
+				} else {
+#pragma unroll
+					for (int k = 0; k < 4; ++k) load_cell<R, E>(g, src, mxb[k], myb, cb[k]);
+				}
+				R Fa[Q], dW[E::NW];
+				bool ok;
+				edge_flux<R, E, PLM, LIM, FLIM, SLIM, RM>(par, lim, slim, cb, dt_dx0, flux_on<E>(g, 0),
+					flux_aov<E>(g, 0), Fa, dW, ok);
+				edges[0].ok[lane] = ok ? 1 : 0;
+#pragma unroll
+				for (int q = 0; q < Q; ++q) edges[0].fa[q][lane] = Fa[q];
+				if constexpr (XE::U) {
+#pragma unroll
+					for (int f = 0; f < C::NV; ++f) {
+						edges[0].u[f][lane] = cb[1].v[f];
+						edges[1].u[f][lane] = cb[2].v[f];
+					}
+				}
+				if constexpr (LIM) {
+#pragma unroll
+					for (int q = 0; q < E::NW; ++q) edges[0].dw[q][lane] = dW[q];
+				}
+			}
+			__syncthreads();
 		} else {
+			const int bj = lane & (EDGE_ROWS - 1);
+			const bool right = lane >= EDGE_ROWS;
+			const int yb = y0 + (bj < y1 - y0 ? bj : 0);
+			const int xb = sx * STRIP_OUT + (right ? STRIP_OUT : 0);   // interface xb - 1/2
+			const Mapped myb = DIM >= 2 ? map_coord(yb, ny, g.bc[2], g.bc[3], 2) : myIn;
+			Mapped mxb[4];
+			bool plain = myb.fixedFace < 0 && !myb.mirror;
 #pragma unroll
-			for (int k = 0; k < 4; ++k) load_cell<R, E>(g, src, mxb[k], myb, cb[k]);
+			for (int k = 0; k < 4; ++k) {
+				mxb[k] = map_coord(xb - 2 + k, nx, g.bc[0], g.bc[1], 0);
+				plain = plain && mxb[k].fixedFace < 0 && !mxb[k].mirror;
+			}
+			C cb[4];
+			if (wave_all(plain)) {
+#pragma unroll
+				for (int k = 0; k < 4; ++k) load_plain<R, E>(g, src, (long)myb.idx * g.step1 + mxb[k].idx, cb[k]);
+			} else {
+#pragma unroll
+				for (int k = 0; k < 4; ++k) load_cell<R, E>(g, src, mxb[k], myb, cb[k]);
+			}
+			edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS, RM>(par, lim, slim, cb, dt_dx0, flux_on<E>(g, 0),
+				flux_aov<E>(g, 0), lane, edge);
 		}
-		edge_pass<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS, RM>(par, lim, slim, cb, dt_dx0, flux_on<E>(g, 0),
-			flux_aov<E>(g, 0), lane, edge);
 	}
 
 	// y pipeline state
@@ -957,8 +1011,10 @@ __device__ __forceinline__ double stage_strip(const StageArgs<R> &a, const GridD
 			C Um, Up;
 			R deriv[Q];
 			bool okX;
+			const int jr = r - y0;
 			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_ROWS, EDGES, RM>(par, lim, slim, U, dt_dx0, flux_on<E>(g, 0),
-				flux_aov<E>(g, 0), lane, r - y0, edge, Um, Up, deriv, okX);
+				flux_aov<E>(g, 0), lane, share ? w * rpw + jr : jr, share ? (w + 1) * rpw + jr : EDGE_ROWS + jr, edge,
+				edgeR, Um, Up, deriv, okX);
 			if constexpr (RM == ROE_FLAGGED) {
 				// an irregular interface of this row: the redo kernel takes the
 				// strip from this row on; nothing of it is stored here
@@ -1098,7 +1154,12 @@ __global__ void __launch_bounds__(256) stage_kernel(const StageArgs<R> a)
 	const int sx = (int)(item % g.stripsX);
 	const int sy = (int)(item / g.stripsX);
 	if (item == 0) stage_first_strip(a, g, dt, lane);
-	double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, RM>(a, g, sx, sy, dt, lane, edges[waveInBlock]);
+	// shared edges: the block's four waves are strips sx0 .. sx0+3 of one row
+	// band (the host sets the flag only where the strips per row are a
+	// multiple of 4, so every block is whole)
+	const bool share = DIM == 2 && strip_edges(DIM, EQ<R>::SOURCE) && a.shareEdges != 0 && blockDim.x == 256;
+	double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, RM>(a, g, sx, sy, dt, lane, edges, waveInBlock,
+		share);
 	stage_reduce(a, g, dtLocal, item, lane);
 }
 
@@ -1140,7 +1201,7 @@ __global__ void __launch_bounds__(256) stage_kernel_redo(const StageArgs<R> a)
 				const int2 e = g.redo[k - base];
 				const int sx = e.x % g.stripsX, sy = e.x / g.stripsX;
 				double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, ROE_INLINE>(a, g, sx, sy, dt, lane,
-					edges[waveInBlock], e.y);
+					edges, waveInBlock, false, e.y);
 				stage_reduce(a, g, dtLocal, e.x, lane);
 			}
 			base += n;
@@ -1194,7 +1255,8 @@ __global__ void __launch_bounds__(256) stage_kernel_batch(const StageArgs<R> a)
 		const int sx = (int)(local % g.stripsX);
 		const int sy = (int)(local / g.stripsX);
 		if (local == 0) stage_first_strip(a, g, dt, lane);
-		double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, RM>(a, g, sx, sy, dt, lane, edges[waveInBlock]);
+		double dtLocal = stage_strip<R, EQ, DIM, PLM, LIM, FLIM, SLIM, RM>(a, g, sx, sy, dt, lane, edges, waveInBlock,
+			false);
 		stage_reduce(a, g, dtLocal, local, lane);
 	}
 }

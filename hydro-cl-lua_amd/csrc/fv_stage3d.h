@@ -277,7 +277,7 @@ __device__ __forceinline__ double stage_rows3d(const StageArgs<R> &a, const Grid
 			R deriv[Q];
 			bool okx;   // unused: the 3D kernel takes the general Roe form where an interface is irregular (ROE_INLINE)
 			strip_x<R, E, PLM, LIM, FLIM, SLIM, EDGE_PLANES>(par, lim, slim, U, dt_dx0, flux_on<E>(g, 0),
-				flux_aov<E>(g, 0), lane, r - z0, edge, Um, Up, deriv, okx);
+				flux_aov<E>(g, 0), lane, r - z0, EDGE_PLANES + (r - z0), edge, edge, Um, Up, deriv, okx);
 #pragma unroll
 			for (int j = 0; j < Q; ++j) {
 				deriv[j] = deriv[j] - dyterm[j];

@@ -161,6 +161,11 @@ static int solver_stage(hydro_solver *h, int s, int useSlot, double hostDt, int 
 	a.advanceTime = advance;
 	a.xcdRemap = h->xcdRemap;
 	a.nBlocks = (int)(g.dim == 3 ? a.g.nItems : (a.g.nItems + 3) / 4);
+	// a block's four waves are x-adjacent strips of one row band when the
+	// strips per row are a multiple of 4: one edge pass computes the five
+	// edge columns they need (5 rows-per-wave interfaces in one wave)
+	a.shareEdges = g.dim == 2 && strip_edges(2, eqn_has_source(g.cfg.eqn)) && a.g.stripsX % 4 == 0 &&
+		5 * a.g.rowsPerWave <= WAVE;
 	const bool redo = redo_kernels(g.cfg);
 	if (ev0) HIPCHECK(hipEventRecord(ev0, h->stream));
 	const StageKey k = stage_key(g);
@@ -169,6 +174,7 @@ static int solver_stage(hydro_solver *h, int s, int useSlot, double hostDt, int 
 	if (redo) {
 		// the strips the stage kernel left to the general Roe form (usually none)
 		a.redoPass = 1;
+		a.shareEdges = 0;
 		a.nBlocks = REDO_BLOCKS;
 		e = launch_stage<R>(k, a, dim3(REDO_BLOCKS), dim3(256), h->stream);
 		if (e != hipSuccess) return fail("redo kernel launch: %s", hipGetErrorString(e));

@@ -188,3 +188,37 @@ def test_redo_same_bits_any_launch(name, real):
     for U, t in outs[1:]:
         assert t == outs[0][1]
         assert np.array_equal(U, outs[0][0])
+
+
+@pytest.mark.parametrize('real', ['double', 'float'])
+def test_shared_edges_same_bits(real):
+    """Where the strips per row are a multiple of 4 the plain kernel's blocks
+    share one strip-edge pass (4 and 12 rows per wave here: five edge columns
+    of rows-per-wave interfaces fit one wave); at 13 and 16 rows, and in the
+    batched kernel, each wave computes its own. The interfaces are the same
+    arithmetic either way, so the results are the same to the bit, with the
+    vacuum expansion's irregular strips going through the redo kernel."""
+    from hydro.solver.batch import HipBatch
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as init_euler
+    init = _expansion(0)
+    kw = dict(dim=2, gridSize=(256, 40), **RK2, real=real, fastMath=True)
+    steps = 20
+    outs = []
+    for rows in (4, 12, 13, 16):
+        s = HipFVSolver(**kw)
+        s.setLaunch(rows)
+        U0 = init_euler.to_ghosted(s.cfg, init(s.cfg)).astype(s.dtype)
+        s.resetState(U0)
+        s.update(steps)
+        outs.append((s.UBufObj.toCPU(), s.t))
+        del s
+    ckw = dict(oracle_kw(kw), precision=real, fast_math=True)
+    b = HipBatch([ckw])
+    b.upload(0, U0)
+    b.constrainU()
+    b.update(steps)
+    outs.append((b.download(0), b.time(0)[0]))
+    for U, t in outs[1:]:
+        assert t == outs[0][1]
+        assert np.array_equal(U, outs[0][0])

@@ -248,6 +248,23 @@ static long device_cus() {
 	return cus;
 }
 
+static StageKey stage_key_cfg(const hydro_cfg &cfg, int nvel);
+
+// Blocks of the plain FP64 stage kernel of cfg that a CU holds at once (its
+// registers and LDS decide: the heavier reconstructions and fluxes hold 2 or
+// 3 where 'plm cons' with Roe holds 4); 4 if the query fails. The velocity
+// components are taken as the dimension's, the usual Euler state.
+static long plain_occupancy(const hydro_cfg &cfg) {
+	StageArgs<double> a{};
+	const StageKey k = stage_key_cfg(cfg, std::min(cfg.dim, 3));
+	stage_occupancy = 0;
+	if (launch_stage<double>(k, a, dim3(0), dim3(0), nullptr) != hipSuccess || stage_occupancy < 1) {
+		(void)hipGetLastError();
+		return 4;
+	}
+	return stage_occupancy;
+}
+
 // Rows per wave of the plain stage kernel (2D), from KH sweeps (rows 3..32 at
 // 512^2 .. 4096^2, fast math). Each wave marches `rows` rows of a 64-column
 // strip after a warm-up of 1-3 rows and its strip-edge pass (about 1.5 rows of
@@ -285,6 +302,7 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 		return (int)std::max(4L, std::min(8L, (stripsX * (long)ny + 4095) / 4096));
 	static const double eff[5] = {0, .7, .77, .9, 1.};
 	const long cus = device_cus();
+	const long occ = plain_occupancy(cfg);
 	int best = 4;
 	double bestT = -1;
 	// at most 8 rows: longer strips measured no faster at 2048^2 and 4096^2
@@ -293,7 +311,7 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 		long L = (blocks + cus - 1) / cus;
 		// the edge pass: a wave's own, or a quarter of its block's shared one
 		const double edge = stripsX % 4 == 0 && 5 * r <= WAVE ? .5 : 1.5;
-		double t = (double)L * (r + warm + edge) / eff[std::min(L, 4L)];
+		double t = (double)L * (r + warm + edge) / eff[std::min(std::min(L, occ), 4L)];
 		if (bestT < 0 || t < bestT * (1 - 1e-9)) { best = r; bestT = t; }
 	}
 	return best;
@@ -484,21 +502,23 @@ static void stage_args(const Grid &g, int s, StageArgs<R> &a) {
 	a.shareEdges = 0;
 }
 
-static StageKey stage_key(const Grid &g) {
+static StageKey stage_key_cfg(const hydro_cfg &cfg, int nvel) {
 	StageKey k;
-	k.eqn = g.cfg.eqn;
-	k.dim = g.dim;
-	k.plm = g.cfg.plm != HYDRO_PLM_NONE;
-	k.flux = g.cfg.flux;
-	k.roeFix = g.cfg.roe_entropy_fix != 0;
-	k.plmKind = g.cfg.plm;
+	k.eqn = cfg.eqn;
+	k.dim = cfg.dim;
+	k.plm = cfg.plm != HYDRO_PLM_NONE;
+	k.flux = cfg.flux;
+	k.roeFix = cfg.roe_entropy_fix != 0;
+	k.plmKind = cfg.plm;
 	// only the Roe flux uses a flux limiter (fvsolver.lua:63)
-	k.lim = !k.plm && k.flux == HYDRO_FLUX_ROE && g.cfg.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
-	k.limiter = k.plm ? g.cfg.slope_limiter : g.cfg.flux_limiter;
-	k.fast = g.cfg.fast_math != 0;
-	k.nvel = g.nvel;
+	k.lim = !k.plm && k.flux == HYDRO_FLUX_ROE && cfg.flux_limiter != HYDRO_LIMITER_DONOR_CELL;
+	k.limiter = k.plm ? cfg.slope_limiter : cfg.flux_limiter;
+	k.fast = cfg.fast_math != 0;
+	k.nvel = nvel;
 	return k;
 }
+
+static StageKey stage_key(const Grid &g) { return stage_key_cfg(g.cfg, g.nvel); }
 
 // ------------------------------------------------------- layout conversion
 // Ghost coordinate -> interior source, following the reference's x, y, z

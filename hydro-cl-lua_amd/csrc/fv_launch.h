@@ -38,9 +38,11 @@ inline int limiter_template_id(int index) {
 template <typename R> struct GridArgs;
 
 // grid.x > 0: plain launch of that many blocks; grid.x == 0: persistent
-// batched launch over a.grids.
+// batched launch over a.grids; grid.x == 0 and block.x == 0: no launch, the
+// plain kernel's resident blocks per CU go to stage_occupancy (this thread's).
 template <typename R>
 hipError_t launch_stage(const StageKey &k, const StageArgs<R> &a, dim3 grid, dim3 block, hipStream_t s);
+extern thread_local int stage_occupancy;
 // the same for one math policy and dimension (launch_stage picks by k.fast and
 // k.dim); each is defined in a translation unit of its own
 template <typename R, bool FAST, int DIM>

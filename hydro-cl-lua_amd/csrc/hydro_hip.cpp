@@ -24,6 +24,8 @@
 
 using namespace hydro;
 
+thread_local int hydro::stage_occupancy = 0;
+
 // ------------------------------------------------------------------ errors
 std::string &hydro::hydro_err_buf() {
 	static thread_local std::string e;

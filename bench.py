@@ -21,6 +21,8 @@ Beside it on the same JSON line:
   reconstructions  the same KH problem, Roe flux, with 'plm prim', 'plm eig',
                 'plm cons with flux', 'plm athena', 'ppm', 'plm eig prim' and
                 'plm eig prim ref'
+  equations     shallow water (humps on a sloped, bumpy depth) and GLM-Maxwell
+                (Gaussian pulses) on the same grid and scheme
   batched       512 independent 2D Euler four-quadrant grids (sides log-uniform
                 in [64, 512]), FP32, RK2-TVD, 300 steps: one persistent launch
                 per stage over all grids, hipGraph replay, autotuned launch
@@ -158,6 +160,37 @@ def timed_steps(s, warmup, steps):
     kt = s.timeUpdates(steps, perKernel=False)     # events on the solver stream around update(steps)
     s.sync()
     return time.perf_counter() - t0, kt.total_ms
+
+
+def equations_leg(n, steps, warmup, fast):
+    """the other equations on the headline's shape and scheme (n^2, FP64, Roe +
+    plm cons/minmod, RK2-TVD, CFL 0.4): shallow water (seeded humps on a
+    sloped, bumpy depth, freeflow) and GLM-Maxwell (seeded Gaussian pulses,
+    sigma 0, periodic). Algorithmic bytes per cell-update: 5 q w for the q
+    integrated fields, as for Euler, plus the read-only fields read once per
+    stage (the depth; GLM's rhoCharge, sigma, sqrt_1_eps, sqrt_1_mu)."""
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as init_euler, shallow_water as isw, glm_maxwell as iglm
+    legs = {
+        'shallow_water': (4, 1, 'freeflow', lambda cfg: isw.humps(cfg, seed=0)),
+        'glm_maxwell': (8, 4, 'periodic', lambda cfg: iglm.gaussian_pulses(cfg, seed=0, conductivity=0.)),
+    }
+    out = {}
+    for eqn, (q, nstat, bc, init) in legs.items():
+        s = HipFVSolver(dim=2, gridSize=(n, n), mins=(-.5, -.5), maxs=(.5, .5), eqn=eqn, flux='roe',
+                        usePLM='plm cons', slopeLimiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4,
+                        boundary=bc, real='double', fastMath=fast)
+        s.resetState(init_euler.to_ghosted(s.cfg, init(s.cfg)).astype(s.dtype))
+        wall, ev = timed_steps(s, warmup, steps)
+        avg = ev / (2 * steps)
+        per_launch = (5 * q + 2 * nstat) * 8 / 2 * n * n
+        U = interior(s.UBufObj.toCPU(), 2)[..., :q]
+        out[eqn] = {'value': n * n * steps / wall / 1e9, 'unit': 'Gcell-updates/s', 'steps': steps,
+                    'boundary': bc, 'avg_launch_ms': avg, 'algorithmic_bytes_per_launch': per_launch,
+                    'roofline_frac': per_launch / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    'rows_per_wave': s.getLaunch()[0], 't_sim': s.t, 'finite': bool(np.isfinite(U).all())}
+        del s
+    return out
 
 
 def committed_traffic(key):
@@ -551,6 +584,7 @@ def main():
                              'steps': steps_f, 'avg_launch_ms': avg_f,
                              'roofline_frac': 5 * NQ * 8 / 2 * cells / (avg_f * 1e-3) / 1e9 / HBM_PEAK_GBS}
             del sf
+        out['equations'] = equations_leg(n, max(20, args.steps // 5), args.warmup, fast)
     if not args.no_batch:
         out['batched'] = batched_leg(args.batch_grids, args.batch_steps)
     if not args.no_3d:

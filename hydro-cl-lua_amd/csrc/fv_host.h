@@ -266,15 +266,20 @@ static long plain_occupancy(const hydro_cfg &cfg) {
 }
 
 // Rows per wave of the plain stage kernel (2D), from KH sweeps (rows 3..32 at
-// 512^2 .. 4096^2, fast math). Each wave marches `rows` rows of a 64-column
-// strip after a warm-up of 1-3 rows and its strip-edge pass (about 1.5 rows of
-// work; about 0.5 when a block's four x-adjacent strips share one pass).
-// FP64 holds 4 waves per SIMD: blocks of 4 waves are dealt to the CUs, a CU
-// runs L = ceil(blocks / CUs) of them, and the launch takes about
-// L * (rows + warm + edge) / eff(L), eff(L) (fitted: .7, .77, .9, 1 for L = 1,
-// 2, 3, 4+) being how well L waves per SIMD hide latency; the rows in 4..8
-// with the least time are taken: 4 at 512^2 and 1024^2 (one round of 4 waves
-// per SIMD), 8 at 2048^2 and 4096^2. FP32 is latency-bound and gains from many short waves:
+// 512^2 .. 4096^2, fast math). Each wave marches `rows` rows of a strip after a
+// warm-up of 1-3 rows and, in the 64-column strips, its strip-edge pass (about
+// 1.5 rows of work; about 0.5 when a block's four x-adjacent strips share one
+// pass); halo strips (equations with a source) have no edge pass.
+// FP64: blocks of 4 waves are dealt to the CUs, a CU runs L = ceil(blocks /
+// CUs) of them, at most occ at once (the kernel's resident blocks per CU, from
+// an occupancy query), and the launch takes about
+// L * (rows + warm + edge) / eff(min(L, occ)), eff (fitted: .7, .77, .9, 1 for
+// 1, 2, 3, 4+) being how well that many waves per SIMD hide latency; the rows
+// in 4..8 with the least time are taken: for the kernels of 4 waves per SIMD 4
+// at 512^2 and 1024^2 (one round), 8 at 2048^2 and 4096^2. A halo-strip kernel
+// of at most 2 blocks per CU takes rows up to 16, so that its launch can end in
+// fewer rounds (GLM-Maxwell: 10 rows at 1024^2, 12 at 2048^2, each the fastest
+// of a sweep of 3..16). FP32 is latency-bound and gains from many short waves:
 // rows = clamp(strips * ny / 4096, 4, 8) (4 at 512^2 .. 1024^2, 8 from 2048^2;
 // longer waves leave a costly partial last round).
 // 3D: planes per block chunk (at most 16). Blocks are small (4 waves) and many resident
@@ -303,14 +308,17 @@ static int default_rows(const hydro_cfg &cfg, int nx, int ny, int nz) {
 	static const double eff[5] = {0, .7, .77, .9, 1.};
 	const long cus = device_cus();
 	const long occ = plain_occupancy(cfg);
+	const bool edges = strip_edges(cfg.dim, eqn_has_source(cfg.eqn));
 	int best = 4;
 	double bestT = -1;
-	// at most 8 rows: longer strips measured no faster at 2048^2 and 4096^2
-	for (int r = 4; r <= 8; ++r) {
+	// at most 8 rows for 64-column strips and for kernels of 3 or more blocks
+	// per CU: longer strips measured no faster at 2048^2 and 4096^2
+	const int rMax = !edges && occ <= 2 ? 16 : 8;
+	for (int r = 4; r <= rMax; ++r) {
 		long blocks = (stripsX * ((ny + r - 1) / r) + 3) / 4;
 		long L = (blocks + cus - 1) / cus;
 		// the edge pass: a wave's own, or a quarter of its block's shared one
-		const double edge = stripsX % 4 == 0 && 5 * r <= WAVE ? .5 : 1.5;
+		const double edge = !edges ? 0. : stripsX % 4 == 0 && 5 * r <= WAVE ? .5 : 1.5;
 		double t = (double)L * (r + warm + edge) / eff[std::min(std::min(L, occ), 4L)];
 		if (bestT < 0 || t < bestT * (1 - 1e-9)) { best = r; bestT = t; }
 	}

@@ -148,11 +148,15 @@ struct Eqn {
 			+ nl<S, 2, 3>() * tmp27 + e.v[2] * tmp37 + nl<S, 3, 3>() * tmp31)) * inv53;
 	}
 
-	// fluxFromCons (shallow-water.cl:186-206), extraTermInFlux = false
+	// fluxFromCons (shallow-water.cl:186-206), extraTermInFlux = false. The
+	// fast policy takes v_n from the cell's prims (m_n * (1 / h), one rounding
+	// from m_n / h) instead of a second division: 2 fewer divisions per
+	// interface and 21 fewer VGPRs in the kernel without 'plm cons' (139 ->
+	// 118, 4 waves per SIMD)
 	template <int S>
-	__device__ static __forceinline__ void flux(const Par &p, const R *U, const Prim &, R *F) {
+	__device__ static __forceinline__ void flux(const Par &p, const R *U, const Prim &W, R *F) {
 		const R m_n = U[1 + S];
-		const R v_n = M::div(m_n, U[0]);
+		const R v_n = FAST ? W.v[S] : M::div(m_n, U[0]);
 		const R hP = R(.5) * p.g * U[0] * (U[0]);
 		F[0] = m_n;
 		F[1] = U[1] * v_n + nl<S, 1, 1>() * hP;

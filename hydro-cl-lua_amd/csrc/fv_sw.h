@@ -5,7 +5,8 @@
 // arithmetic is the same). extraTermInFlux = false, as the reference sets it.
 // The transforms keep the reference's expressions term for term, including
 // the products with the normal components (constants 0 and 1 here), so the
-// EXACT policy's arithmetic is the reference's.
+// EXACT policy's arithmetic is the reference's; the FAST policy's transforms,
+// flux and source drop the products with a zero component.
 #pragma once
 
 #include "fv_common.h"
@@ -81,9 +82,28 @@ struct Eqn {
 	template <int S, int J, int I>
 	__device__ static constexpr R nl() { return (((I - J) % 3 + 3) % 3) == S ? R(1) : R(0); }
 
-	// eigen_leftTransform (shallow-water.cl:286-318), nLen = 1
+	// The tangential momentum fields of axis S: T1 the one with
+	// normal_l<2> = 1, T2 the one with normal_l<3> = 1 (1-based cons indices)
+	template <int S> static constexpr int T1 = (S + 1) % 3 + 1;
+	template <int S> static constexpr int T2 = (S + 2) % 3 + 1;
+
+	// eigen_leftTransform (shallow-water.cl:286-318), nLen = 1. The fast
+	// policy drops the terms whose normal component is 0 (products with an
+	// exact zero), keeping the order of the others
 	template <int S>
 	__device__ static __forceinline__ void left(const Par &, const Eig &e, const R *X, R *Y) {
+		if constexpr (FAST) {
+			const R v_n = e.v[S];
+			const R invDenom = M::rcp(R(2) * e.h * e.C);
+			const R tmp2 = e.C * X[0];
+			const R tmp5 = -(v_n * X[0]);
+			const R tmp31 = invDenom * e.C;
+			Y[0] = invDenom * (X[1 + S] + tmp5 + -tmp2);
+			Y[1] = R(2) * tmp31 * (X[T1<S>] + -(e.v[T1<S> - 1] * X[0]));
+			Y[2] = R(2) * tmp31 * (X[T2<S>] + -(e.v[T2<S> - 1] * X[0]));
+			Y[3] = invDenom * (X[1 + S] + tmp2 + tmp5);
+			return;
+		}
 		const R nLen = R(1);
 		const R v_n = e.v[S];
 		const R invDenom = M::rcp(R(2) * nLen * e.h * e.C);
@@ -110,6 +130,21 @@ struct Eqn {
 	// eigen_rightTransform (shallow-water.cl:322-362), nLen = n2Len = n3Len = 1
 	template <int S>
 	__device__ static __forceinline__ void right(const Par &, const Eig &e, const R *X, R *Y) {
+		if constexpr (FAST) {
+			// the same sums without their zero normal terms; the lengths are 1
+			const R h = e.h, C = e.C;
+			const R invDenom = M::rcp(C);
+			const R tmp10 = C * C * X[0];
+			const R tmp16 = X[3] * (C * C);
+			const R tmp22 = C * X[0];
+			const R tmp37 = C * X[3];
+			const R vn = e.v[S], v1 = e.v[T1<S> - 1], v2 = e.v[T2<S> - 1];
+			Y[0] = invDenom * (X[3] + -X[0]) * h * C;
+			Y[1 + S] = invDenom * (h * (tmp10 + tmp16 + -vn * tmp22 + vn * tmp37));
+			Y[T1<S>] = invDenom * (h * (-v1 * tmp22 + C * X[1] + v1 * tmp37));
+			Y[T2<S>] = invDenom * (h * (-v2 * tmp22 + v2 * tmp37 + C * X[2]));
+			return;
+		}
 		const R nLen = R(1), n2Len = R(1), n3Len = R(1);
 		const R h = e.h, C = e.C;
 		const R invDenom = M::rcp(C);
@@ -158,6 +193,12 @@ struct Eqn {
 		const R m_n = U[1 + S];
 		const R v_n = FAST ? W.v[S] : M::div(m_n, U[0]);
 		const R hP = R(.5) * p.g * U[0] * (U[0]);
+		if constexpr (FAST) {
+			F[0] = m_n;
+#pragma unroll
+			for (int i = 1; i <= 3; ++i) F[i] = i == 1 + S ? U[i] * v_n + hP : U[i] * v_n;
+			return;
+		}
 		F[0] = m_n;
 		F[1] = U[1] * v_n + nl<S, 1, 1>() * hP;
 		F[2] = U[2] * v_n + nl<S, 1, 2>() * hP;
@@ -198,7 +239,7 @@ struct Eqn {
 			if (j < DIM) {
 				const R partial = (R(0) + (ep[j] - em[j]) * R(1)) * g.inv2dx[j];
 				d[1 + j] = d[1 + j] + partial * gh;
-			} else {
+			} else if constexpr (!FAST) {
 				d[1 + j] = d[1 + j] + R(0) * gh;
 			}
 		}

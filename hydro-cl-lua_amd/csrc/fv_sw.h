@@ -185,9 +185,8 @@ struct Eqn {
 
 	// fluxFromCons (shallow-water.cl:186-206), extraTermInFlux = false. The
 	// fast policy takes v_n from the cell's prims (m_n * (1 / h), one rounding
-	// from m_n / h) instead of a second division: 2 fewer divisions per
-	// interface and 21 fewer VGPRs in the kernel without 'plm cons' (139 ->
-	// 118, 4 waves per SIMD)
+	// from m_n / h) instead of a second division (2 fewer divisions per
+	// interface) and leaves out the zero normal terms
 	template <int S>
 	__device__ static __forceinline__ void flux(const Par &p, const R *U, const Prim &W, R *F) {
 		const R m_n = U[1 + S];

@@ -366,3 +366,32 @@ def test_glm_maxwell_ppm_fast_fp64(dim, n):
     err = rel_l1(_interior(dim, s.UBufObj.toCPU())[..., :8], _interior(dim, o.U)[..., :8])
     print('glm ppm fast', dim, err)
     assert np.all(err <= TOL64), err
+
+
+@pytest.mark.parametrize('eqn', ['glm_maxwell', 'shallow_water'])
+def test_halo_strips_rows_same_bits(eqn):
+    """the equations with a source march halo strips (60 output columns of
+    64 lanes); a grid advances to the same bits at any rows per wave,
+    including the long strips the host picks for GLM-Maxwell FP64 (10-16):
+    three strips per row (the last partial), FAST FP64, 12 RK2-TVD steps"""
+    from hydro.solver.hipfvsolver import HipFVSolver
+    from hydro.init import euler as init_euler, glm_maxwell as iglm, shallow_water as isw
+    if eqn == 'glm_maxwell':
+        init, bc = (lambda cfg: iglm.gaussian_pulses(cfg, seed=3, conductivity=0.)), 'periodic'
+    else:
+        init, bc = (lambda cfg: isw.humps(cfg, seed=3)), 'freeflow'
+    out = {}
+    for rows in (3, 7, 10, 16):
+        s = HipFVSolver(dim=2, gridSize=(130, 70), mins=(-.5, -.5), maxs=(.5, .5), eqn=eqn, usePLM='plm cons',
+                        slopeLimiter='minmod', integrator='Runge-Kutta 2, TVD', cfl=.4, boundary=bc,
+                        real='double', fastMath=True)
+        s.setLaunch(rows, 0)
+        s.resetState(init_euler.to_ghosted(s.cfg, init(s.cfg)).astype(s.dtype))
+        s.update(12)
+        out[rows] = (s.UBufObj.toCPU(), s.t)
+        del s
+    U3, t3 = out[3]
+    assert np.isfinite(U3).all()
+    for rows, (U, t) in out.items():
+        assert t == t3, rows
+        assert np.array_equal(U, U3), rows
